@@ -1,0 +1,197 @@
+"""Benchmark of the sparse upper-triangular solve on one MI355X.
+
+Default workload: U = IC(0) factor of the 7-point Laplacian on a 128^3 grid (n = 2,097,152,
+nnz = 8,339,456), FP64, one right-hand side, b = U x_true with x_true ~ U(-1,1) (seed 1) --
+the single-GPU sync-free configuration the headline metric is quoted on.  One step = one full
+solve (sentinel fill + ticket reset + solve kernel) with U, b and x resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant level_pull] [--m 1]
+
+Prints ONE JSON line.  `value` is achieved GB/s over whole steps, with the algorithmic byte
+count  nnz*(8+4) + 4*(n+1) + 8*n*m (b) + 8*n*m (x)  (index and value of every stored entry, the
+row pointers, b read once, x written once).  `roofline.achieved` divides the same bytes by the
+solve kernel's own average duration, timed with HIP events on the plan's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(n, nnz, m, vbytes=8):
+    return nnz * (vbytes + 4) + 4 * (n + 1) + 2 * vbytes * n * m
+
+
+def make_workload(name):
+    from sptrsv_amd import gen
+    if name == "ic0_128":
+        return gen.laplace3d_ic0(128), "IC(0) of 7-point Laplacian, 128^3 grid"
+    if name == "zipf_1e6":
+        return gen.random_zipf(1_000_000, seed=1), "random upper, n=1e6, Poisson(8) rows, Zipf(1.5) offsets"
+    if name == "chol2d_100":
+        return gen.laplace2d_cholesky(100), "exact Cholesky of 5-point Laplacian, 100^2 grid"
+    if name == "supernodal_4096":
+        return gen.random_supernodal(4096, seed=2), "random supernodal, 4096 supernodes, widths U[16,128]"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def time_solves(capi, P, d_b, d_x, m, prm, steps, warmup):
+    for _ in range(warmup):
+        P.solve(d_b, d_x, m, prm)
+    P.sync()
+    capi.device_sync()
+    P.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.solve(d_b, d_x, m, prm)
+    P.sync()
+    capi.device_sync()
+    t1 = time.perf_counter()
+    kms, launches = P.timing_read()
+    P.timing(False)
+    return (t1 - t0) * 1000.0 / steps, kms / max(launches, 1)
+
+
+def read_traffic(workload, variant, m):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        table = json.load(open(path))
+        return table.get(f"{workload}/{variant}/m{m}")
+    except Exception:
+        return None
+
+
+def cpu_baseline(U, m, budget_s):
+    """The oracle's serial solve (a restatement of the reference's base_gauss_upper) on one
+    host core, repeated until `budget_s` seconds of solve time or 50 solves."""
+    from oracle import oracle as O
+    from sptrsv_amd import gen
+    xt, b = gen.rhs_for(U, m, seed=1, layout="col")
+    times = []
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s and len(times) < 50:
+        _, t = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
+        times.append(t)
+    best = min(times)
+    _, ta, ts = O.barrier_upper(U.rowptr, U.col, U.val, b, O.max_threads())
+    return {
+        "value": alg_bytes(U.n, U.nnz, m) / best / 1e9,
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(times)} full serial solves of the same factor (m={m}), min time {best * 1e3:.2f} ms",
+        "serial_ms": best * 1e3,
+        "openmp_levelset_ms": (ta + ts) * 1e3,
+        "openmp_levelset_analysis_ms": ta * 1e3,
+        "openmp_threads": O.max_threads(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ic0_128")
+    ap.add_argument("--variant", default="level_pull")
+    ap.add_argument("--m", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the solve path here is single-GPU "
+              "(multi-GPU sharding is design-only for this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from oracle import oracle as O
+    from sptrsv_amd import capi, gen
+
+    names = {v: k for k, v in capi.VARIANT_NAMES.items()}
+    variant = names[args.variant]
+    U, desc = make_workload(args.workload)
+    m = args.m
+    P = capi.Plan.from_upper(U)
+    xt, b = gen.rhs_for(U, m, seed=1, layout="row")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    prm = capi.params(variant=variant)
+    ms, kms = time_solves(capi, P, d_b, d_x, m, prm, args.steps, args.warmup)
+    x = d_x.download()
+    bb = b if m == 1 else np.ascontiguousarray(b.T)
+    ref, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, bb)
+    err = O.rel_error(ref if m == 1 else ref.T, x)
+    B = alg_bytes(U.n, U.nnz, m)
+    gbs = B / (ms / 1e3) / 1e9
+    achieved = B / (kms / 1e3) / 1e9
+    name, cus, mem = capi.device_info()
+
+    extras = {}
+    if not args.no_extras:
+        runs = [("ic0_128", "csc_push", 1), ("ic0_128", "row_pull", 1), ("ic0_128", "level_pull", 32),
+                ("ic0_128", "level_pull", 128), ("zipf_1e6", "row_pull", 1), ("zipf_1e6", "level_pull", 1)]
+        cache = {args.workload: (U, P)}
+        for wl, vname, mm in runs:
+            if (wl, vname, mm) == (args.workload, args.variant, m):
+                continue
+            try:
+                if wl not in cache:
+                    UU, _ = make_workload(wl)
+                    cache[wl] = (UU, capi.Plan.from_upper(UU))
+                UU, PP = cache[wl]
+                _, bx = gen.rhs_for(UU, mm, seed=1, layout="row")
+                db = capi.DeviceArray.from_host(bx)
+                dx = capi.DeviceArray(bx.shape)
+                e_ms, e_kms = time_solves(capi, PP, db, dx, mm, capi.params(variant=names[vname]),
+                                          3 if mm > 1 or wl != "ic0_128" else 10, 1)
+                BB = alg_bytes(UU.n, UU.nnz, mm)
+                extras[f"{wl}/{vname}/m{mm}"] = {"ms_per_step": round(e_ms, 4), "kernel_ms": round(e_kms, 4),
+                                                 "GB_s": round(BB / (e_ms / 1e3) / 1e9, 2),
+                                                 "nnz_per_s": UU.nnz * mm / (e_ms / 1e3)}
+                db.free()
+                dx.free()
+            except Exception as exc:  # report, never hide
+                extras[f"{wl}/{vname}/m{mm}"] = {"error": str(exc)}
+
+    cpu = None if args.no_cpu else cpu_baseline(U, m, args.cpu_budget)
+    out = {
+        "metric": "SpTRSV GB/s & % HBM peak; time-to-1e-12 error, mixed+refine vs FP64 sync-free",
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prm.precision == capi.FP64 else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {desc}", "n": U.n, "nnz": U.nnz, "m": m,
+                   "variant": args.variant, "device": name, "compute_units": cus},
+        "pct_hbm_peak": round(100.0 * gbs / HBM_PEAK_GBS, 3),
+        "nnz_per_s": U.nnz * m / (ms / 1e3),
+        "rel_err_vs_oracle": err,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args.workload, args.variant, m),
+                     "kernel_ms": round(kms, 5), "alg_bytes": B},
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

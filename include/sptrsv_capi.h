@@ -1,0 +1,131 @@
+/*
+ * sptrsv_capi.h -- C ABI of the MI355X (gfx950) sparse upper-triangular solver.
+ *
+ * Plain C types only (pointers, sizes, int status codes).  The C++ drop-in headers
+ * (core.h, utils.h, sptrsv_syncfree_opencl.h) are thin wrappers over these calls, and
+ * any FFI (ctypes, cgo, JNI) can bind them directly.
+ *
+ * Model: a *plan* owns one upper-triangular U resident in device memory (CSR, CSC,
+ * level-ordered CSR, analysis arrays).  Solves take DEVICE pointers to b and x and run
+ * asynchronously on the plan's stream; sptrsv_plan_sync() waits and reports device-side
+ * failures (watchdog).  sptrsv_solve_host() is the synchronous host-pointer convenience.
+ *
+ * Layouts: CSR rows hold the diagonal FIRST and ascending columns (the reference's CPU
+ * convention, src/core.cpp:18-32); CSC columns hold ascending rows with the diagonal LAST
+ * (src/utils.cpp:128-159).  Multi-RHS device arrays are row-major n x m (x[i*m + k], the
+ * reference's GPU SpTRSM layout, src/kernel_full.cl:163) unless a call says otherwise.
+ *
+ * Every function returns SPTRSV_OK (0) or a negative SPTRSV_ERR_* code; the matching
+ * message is available from sptrsv_last_error() on the calling thread.
+ */
+#ifndef SPTRSV_CAPI_H
+#define SPTRSV_CAPI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPTRSV_OK 0
+#define SPTRSV_ERR_BAD_INPUT -1   /* malformed matrix or argument (nothing launched) */
+#define SPTRSV_ERR_HIP -2         /* HIP runtime error */
+#define SPTRSV_ERR_DEADLOCK -3    /* device watchdog: a dependency wait exceeded its limit */
+#define SPTRSV_ERR_IO -4          /* file missing / unknown extension / short read */
+#define SPTRSV_ERR_NOMEM -5       /* device or host allocation failed */
+#define SPTRSV_ERR_UNSUPPORTED -6 /* size or option outside what the kernels support */
+#define SPTRSV_ERR_NO_DEVICE -7   /* no HIP device visible */
+
+/* Solver variants (kernel families). */
+#define SPTRSV_VAR_AUTO 0         /* selector picks */
+#define SPTRSV_VAR_CSC_PUSH 1     /* wave per CSC column, in-degree counters + atomic left_sum
+                                     (the reference's algorithm, src/kernel_full.cl:54-228) */
+#define SPTRSV_VAR_ROW_PULL 2     /* lanes per CSR row in natural (descending) order, x published
+                                     through its own value; exact reference summation order */
+#define SPTRSV_VAR_LEVEL_PULL 3   /* same kernel over level-ordered rows (level-permuted CSR) */
+#define SPTRSV_VAR_LEVELSET 4     /* bulk-synchronous: one launch per level (src/core.cpp:273-376) */
+#define SPTRSV_VAR_WINDOW 5       /* wave per 64 consecutive rows, in-window dependencies resolved
+                                     through cross-lane broadcasts (chains) */
+#define SPTRSV_VAR_COUNT 6
+
+/* Value precision of the solve. */
+#define SPTRSV_FP64 0
+#define SPTRSV_FP32 1
+
+typedef struct {
+    int variant;      /* SPTRSV_VAR_* */
+    int group;        /* lanes per row for the pull kernels (1,2,4,...,64); 0 = derive from m */
+    int unroll;       /* entries polled per step in the pull kernels (1, 2 or 4); 0 = default */
+    int precision;    /* SPTRSV_FP64 / SPTRSV_FP32 */
+    int watchdog_us;  /* per-wait device watchdog in microseconds; 0 = default (20 s) */
+    int reserved[3];
+} sptrsv_params_t;
+
+typedef struct sptrsv_plan_s* sptrsv_plan_t;
+
+/* ---- device / memory ---------------------------------------------------------------- */
+const char* sptrsv_last_error(void);
+const char* sptrsv_status_string(int code);
+int sptrsv_device_init(int device);
+int sptrsv_device_info(char* name, int name_len, int* compute_units, size_t* total_mem);
+int sptrsv_device_sync(void);
+int sptrsv_malloc(void** ptr, size_t bytes);
+int sptrsv_free(void* ptr);
+int sptrsv_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int sptrsv_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int sptrsv_memcpy_d2d(void* dst, const void* src, size_t bytes);
+
+/* ---- host-side validation (no device needed) ---------------------------------------- */
+/* Checks an upper CSR: rowptr[0]==0, monotone, rowptr[n]==nnz, first entry of each row is
+ * its diagonal (finite, non-zero), remaining columns strictly ascending and < n. */
+int sptrsv_validate_csr(int n, int64_t nnz, const int64_t* rowptr, const int* col, const double* val);
+/* Checks an upper CSC: colptr monotone, rows strictly ascending, last entry of each column
+ * is its diagonal (finite, non-zero). */
+int sptrsv_validate_csc(int n, int64_t nnz, const int* colptr, const int* rowidx, const double* val);
+
+/* ---- format conversion (device) ------------------------------------------------------- */
+/* CSR (any pattern, n x n) -> CSC with rows ascending in every column: the reference's
+ * transpose (src/utils.cpp:128-159) as a stable device radix sort.  Host pointers. */
+int sptrsv_csr_to_csc(int n, int64_t nnz, const int64_t* rowptr, const int* col, const double* val,
+                      int* colptr /* n+1 */, int* rowidx /* nnz */, double* cval /* nnz */);
+
+/* ---- plans ---------------------------------------------------------------------------- */
+/* Uploads U (host CSR, validated), builds on the device: CSC (stable transpose), in-degree,
+ * levels, level sets and the level-ordered CSR.  nnz must be < 2^31. */
+int sptrsv_plan_create_csr(int n, int64_t nnz, const int64_t* rowptr, const int* col,
+                           const double* val, sptrsv_plan_t* plan);
+/* Same from a host CSC (diagonal last) -- the reference GPU entry's input. */
+int sptrsv_plan_create_csc(int n, int64_t nnz, const int* colptr, const int* rowidx,
+                           const double* val, sptrsv_plan_t* plan);
+int sptrsv_plan_destroy(sptrsv_plan_t plan);
+int sptrsv_plan_info(sptrsv_plan_t plan, int* n, int64_t* nnz, int* n_levels);
+
+/* Analysis results (device-computed, copied to host). */
+int sptrsv_plan_get_indegree(sptrsv_plan_t plan, int* indeg /* n */);
+int sptrsv_plan_get_levels(sptrsv_plan_t plan, int* levels /* n, 1-based */);
+int sptrsv_plan_get_level_sets(sptrsv_plan_t plan, int* level_ptr /* n_levels+1 */,
+                               int* level_rows /* n */);
+int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /* nnz */,
+                        double* val /* nnz */);
+
+/* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).
+ * Asynchronous on the plan's stream. */
+int sptrsv_solve(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
+                 double* d_x, int m);
+/* Waits for the plan's stream; returns SPTRSV_ERR_DEADLOCK if a watchdog fired. */
+int sptrsv_plan_sync(sptrsv_plan_t plan);
+/* Host pointers, synchronous.  col_major != 0: b and x are column-major (x[i + k*n]). */
+int sptrsv_solve_host(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* b,
+                      double* x, int m, int col_major);
+
+/* HIP-event timing of the main solve kernel of every solve while enabled. */
+int sptrsv_plan_timing(sptrsv_plan_t plan, int enable);
+int sptrsv_plan_timing_read(sptrsv_plan_t plan, double* total_ms, int* launches);
+
+/* Residual r = b - U x on the device, FP64, row-major n x m. */
+int sptrsv_residual(sptrsv_plan_t plan, const double* d_b, const double* d_x, double* d_r, int m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,304 @@
+"""ctypes bindings of libsptrsv_amd.so (the C ABI in include/sptrsv_capi.h).
+
+The shared library is the product: this module only loads it and converts numpy arrays.
+Importing it without a built library raises immediately -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libsptrsv_amd.so")
+
+OK = 0
+ERR_BAD_INPUT = -1
+ERR_HIP = -2
+ERR_DEADLOCK = -3
+ERR_IO = -4
+ERR_NOMEM = -5
+ERR_UNSUPPORTED = -6
+ERR_NO_DEVICE = -7
+
+VAR_AUTO = 0
+VAR_CSC_PUSH = 1
+VAR_ROW_PULL = 2
+VAR_LEVEL_PULL = 3
+VAR_LEVELSET = 4
+VAR_WINDOW = 5
+VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
+                 VAR_LEVELSET: "levelset", VAR_WINDOW: "window"}
+
+FP64 = 0
+FP32 = 1
+
+
+class SptrsvError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: status {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("group", ctypes.c_int), ("unroll", ctypes.c_int),
+                ("precision", ctypes.c_int), ("watchdog_us", ctypes.c_int),
+                ("reserved", ctypes.c_int * 3)]
+
+
+_lib = None
+_vp, _i, _i64, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t
+_PLAN = ctypes.c_void_p
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lib():
+    """Load the shared library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m sptrsv_amd.build` "
+                          "(the solver has no non-GPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "sptrsv_last_error": (ctypes.c_char_p, []),
+        "sptrsv_status_string": (ctypes.c_char_p, [_i]),
+        "sptrsv_device_init": (_i, [_i]),
+        "sptrsv_device_info": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
+        "sptrsv_device_sync": (_i, []),
+        "sptrsv_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
+        "sptrsv_free": (_i, [_vp]),
+        "sptrsv_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+        "sptrsv_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+        "sptrsv_memcpy_d2d": (_i, [_vp, _vp, _sz]),
+        "sptrsv_validate_csr": (_i, [_i, _i64, _vp, _vp, _vp]),
+        "sptrsv_validate_csc": (_i, [_i, _i64, _vp, _vp, _vp]),
+        "sptrsv_csr_to_csc": (_i, [_i, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "sptrsv_plan_create_csr": (_i, [_i, _i64, _vp, _vp, _vp, ctypes.POINTER(_PLAN)]),
+        "sptrsv_plan_create_csc": (_i, [_i, _i64, _vp, _vp, _vp, ctypes.POINTER(_PLAN)]),
+        "sptrsv_plan_destroy": (_i, [_PLAN]),
+        "sptrsv_plan_info": (_i, [_PLAN, ctypes.POINTER(_i), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
+        "sptrsv_plan_get_indegree": (_i, [_PLAN, _vp]),
+        "sptrsv_plan_get_levels": (_i, [_PLAN, _vp]),
+        "sptrsv_plan_get_level_sets": (_i, [_PLAN, _vp, _vp]),
+        "sptrsv_plan_get_csc": (_i, [_PLAN, _vp, _vp, _vp]),
+        "sptrsv_solve": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
+        "sptrsv_plan_sync": (_i, [_PLAN]),
+        "sptrsv_solve_host": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i, _i]),
+        "sptrsv_plan_timing": (_i, [_PLAN, _i]),
+        "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
+        "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, where: str):
+    if rc != OK:
+        raise SptrsvError(rc, where, lib().sptrsv_last_error().decode())
+
+
+def params(variant=VAR_AUTO, group=0, unroll=0, precision=FP64, watchdog_us=0) -> Params:
+    p = Params()
+    p.variant, p.group, p.unroll, p.precision, p.watchdog_us = variant, group, unroll, precision, watchdog_us
+    return p
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cus, mem = _i(0), _sz(0)
+    check(lib().sptrsv_device_info(name, 256, ctypes.byref(cus), ctypes.byref(mem)), "device_info")
+    return name.value.decode(), cus.value, mem.value
+
+
+def device_sync():
+    check(lib().sptrsv_device_sync(), "device_sync")
+
+
+def validate_csr(n, rowptr, col, val) -> int:
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    if len(rp) != n + 1 or len(v) != len(c):
+        return ERR_BAD_INPUT
+    return lib().sptrsv_validate_csr(int(n), len(c), _ptr(rp), _ptr(c), _ptr(v))
+
+
+def validate_csc(n, colptr, rowidx, val) -> int:
+    cp = np.ascontiguousarray(colptr, dtype=np.int32)
+    ri = np.ascontiguousarray(rowidx, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    if len(cp) != n + 1 or len(v) != len(ri):
+        return ERR_BAD_INPUT
+    return lib().sptrsv_validate_csc(int(n), len(ri), _ptr(cp), _ptr(ri), _ptr(v))
+
+
+def last_error() -> str:
+    return lib().sptrsv_last_error().decode()
+
+
+def csr_to_csc(n, rowptr, col, val):
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    nnz = int(rp[-1])
+    cp = np.zeros(n + 1, dtype=np.int32)
+    ri = np.zeros(max(nnz, 1), dtype=np.int32)
+    cv = np.zeros(max(nnz, 1), dtype=np.float64)
+    check(lib().sptrsv_csr_to_csc(int(n), nnz, _ptr(rp), _ptr(c), _ptr(v), _ptr(cp), _ptr(ri), _ptr(cv)),
+          "csr_to_csc")
+    return cp, ri[:nnz], cv[:nnz]
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-shaped array (no torch involved)."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = _vp()
+        check(lib().sptrsv_malloc(ctypes.byref(self.ptr), self.nbytes), "malloc")
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().sptrsv_memcpy_h2d(self.ptr, _ptr(a), self.nbytes), "h2d")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().sptrsv_memcpy_d2h(_ptr(out), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().sptrsv_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Plan:
+    """One upper-triangular factor resident on the GPU (CSR + CSC + analysis)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        n, nnz, L = _i(0), _i64(0), _i(0)
+        check(lib().sptrsv_plan_info(self.handle, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(L)), "info")
+        self.n, self.nnz, self.n_levels = n.value, nnz.value, L.value
+
+    @classmethod
+    def from_csr(cls, rowptr, col, val):
+        rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        c = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        h = _PLAN()
+        check(lib().sptrsv_plan_create_csr(len(rp) - 1, int(rp[-1]), _ptr(rp), _ptr(c), _ptr(v),
+                                           ctypes.byref(h)), "plan_create_csr")
+        return cls(h)
+
+    @classmethod
+    def from_upper(cls, U):
+        return cls.from_csr(U.rowptr, U.col, U.val)
+
+    @classmethod
+    def from_csc(cls, colptr, rowidx, val):
+        cp = np.ascontiguousarray(colptr, dtype=np.int32)
+        ri = np.ascontiguousarray(rowidx, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        h = _PLAN()
+        check(lib().sptrsv_plan_create_csc(len(cp) - 1, int(cp[-1]), _ptr(cp), _ptr(ri), _ptr(v),
+                                           ctypes.byref(h)), "plan_create_csc")
+        return cls(h)
+
+    def close(self):
+        if self.handle:
+            lib().sptrsv_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- analysis ----
+    def indegree(self):
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().sptrsv_plan_get_indegree(self.handle, _ptr(out)), "get_indegree")
+        return out
+
+    def levels(self):
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().sptrsv_plan_get_levels(self.handle, _ptr(out)), "get_levels")
+        return out
+
+    def level_sets(self):
+        ptr = np.zeros(self.n_levels + 1, dtype=np.int32)
+        rows = np.zeros(self.n, dtype=np.int32)
+        check(lib().sptrsv_plan_get_level_sets(self.handle, _ptr(ptr), _ptr(rows)), "get_level_sets")
+        return ptr, rows
+
+    def csc(self):
+        cp = np.zeros(self.n + 1, dtype=np.int32)
+        ri = np.zeros(self.nnz, dtype=np.int32)
+        cv = np.zeros(self.nnz, dtype=np.float64)
+        check(lib().sptrsv_plan_get_csc(self.handle, _ptr(cp), _ptr(ri), _ptr(cv)), "get_csc")
+        return cp, ri, cv
+
+    # ---- solves ----
+    def solve(self, d_b: DeviceArray, d_x: DeviceArray, m: int = 1, prm: Params | None = None):
+        """Asynchronous device solve (row-major n x m)."""
+        check(lib().sptrsv_solve(self.handle, ctypes.byref(prm or params()), d_b.ptr, d_x.ptr, int(m)), "solve")
+
+    def sync(self):
+        check(lib().sptrsv_plan_sync(self.handle), "sync")
+
+    def solve_host(self, b: np.ndarray, prm: Params | None = None, col_major: bool = False):
+        """Synchronous solve of host b: shape (n,), (n, m) row-major, or (m, n) with col_major."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.ndim == 1:
+            m = 1
+        else:
+            m = b.shape[0] if col_major else b.shape[1]
+        x = np.empty_like(b)
+        check(lib().sptrsv_solve_host(self.handle, ctypes.byref(prm or params()), _ptr(b), _ptr(x), int(m),
+                                      int(bool(col_major))), "solve_host")
+        return x
+
+    def residual(self, d_b: DeviceArray, d_x: DeviceArray, d_r: DeviceArray, m: int = 1):
+        check(lib().sptrsv_residual(self.handle, d_b.ptr, d_x.ptr, d_r.ptr, int(m)), "residual")
+
+    def timing(self, enable: bool):
+        check(lib().sptrsv_plan_timing(self.handle, int(enable)), "timing")
+
+    def timing_read(self):
+        ms, cnt = ctypes.c_double(0.0), _i(0)
+        check(lib().sptrsv_plan_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(cnt)), "timing_read")
+        return ms.value, cnt.value

@@ -1,0 +1,300 @@
+// analysis.hip -- device-side analysis of an upper-triangular factor:
+//   CSR -> CSC transpose (stable; diagonal last)          src/utils.cpp:128-159
+//   in-degree per row (incl. diagonal)                     src/kernel_full.cl:40-52
+//   1-based levels (dependency depth)                      src/core.cpp:244-270
+//   level sets + level pointers (rows ascending per level) src/core.cpp:282-287
+//   level-ordered CSR (rows stored level by level)         (no counterpart: feeds the
+//                                                           level-scheduled sync-free kernel)
+// All integer results are exact; the tests compare them bit for bit with the oracle.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace sptrsv {
+
+namespace {
+
+__global__ void k_iota(int* out, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = (int)i;
+}
+
+__global__ void k_fill_int(int* out, int value, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = value;
+}
+
+// row id of every CSR position
+__global__ void k_expand_rows(const int* __restrict__ rowptr, int n, int* __restrict__ row_of) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    for (; r < n; r += gridDim.x * blockDim.x)
+        for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) row_of[p] = r;
+}
+
+// histogram: counts[key[k] + shift]++
+__global__ void k_histogram(const int* __restrict__ keys, int64_t count, int shift, int* __restrict__ counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) atomicAdd(&counts[keys[i] + shift], 1);
+}
+
+__global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict__ row_of,
+                             const double* __restrict__ val, int64_t nnz, int* __restrict__ rowidx,
+                             double* __restrict__ cval) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int src = perm[k];
+        rowidx[k] = row_of[src];
+        cval[k] = val[src];
+    }
+}
+
+// Level of every row by a sync-free max-plus sweep: rows are taken in descending order by
+// wave tickets; a row's level is 1 + the deepest level among its off-diagonal columns (1
+// for a diagonal-only row), exactly the reference's rule.  level 0 marks "not yet known".
+__global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ rowptr,
+                                                const int* __restrict__ col, int* levels,
+                                                int* ctrl, uint64_t limit) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = wave_ticket(&ctrl[0]);
+    const int row = n - 1 - (t * kWave + lane);
+    bool done = row < 0;
+    int p = 0, end = 0, deepest = 1, mine = 0;
+    if (!done) {
+        p = rowptr[row] + 1;
+        end = rowptr[row + 1];
+    }
+    Watchdog wd;
+    wd.arm(&ctrl[1], limit);
+    int idle = 0;
+    while (true) {
+        bool progressed = false;
+        if (!done) {
+            while (p < end) {
+                const int lv = __hip_atomic_load(&levels[col[p]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lv == 0) break;
+                deepest = lv > deepest ? lv : deepest;
+                ++p;
+                progressed = true;
+            }
+            if (p >= end) {
+                const int lev = (end - (rowptr[row] + 1) > 0) ? 1 + deepest : 1;
+                __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                mine = lev;
+                done = true;
+                progressed = true;
+            }
+        }
+        if (__all(done)) break;
+        if (!__any(progressed)) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
+        }
+    }
+    // one atomic per wave for the maximum level
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mine, off, kWave);
+        mine = o > mine ? o : mine;
+    }
+    if (lane == 0) atomicMax(&ctrl[2], mine);
+}
+
+__global__ void k_level_rowlen(const int* __restrict__ rowptr, const int* __restrict__ level_rows, int n,
+                               int* __restrict__ lens) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < n; k += gridDim.x * blockDim.x) {
+        const int r = level_rows[k];
+        lens[k + 1] = rowptr[r + 1] - rowptr[r];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) lens[0] = 0;
+}
+
+// copy row level_rows[k] into slot k of the level-ordered CSR; one wave per row
+__global__ void k_level_gather(const int* __restrict__ rowptr, const int* __restrict__ col,
+                               const double* __restrict__ val, const int* __restrict__ level_rows,
+                               const int* __restrict__ lrowptr, int n, int* __restrict__ lcol,
+                               double* __restrict__ lval) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int k = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int stride = gridDim.x * blockDim.x / kWave;
+    for (; k < n; k += stride) {
+        const int r = level_rows[k];
+        const int a = rowptr[r], len = rowptr[r + 1] - a, d = lrowptr[k];
+        for (int e = lane; e < len; e += kWave) {
+            lcol[d + e] = col[a + e];
+            lval[d + e] = val[a + e];
+        }
+    }
+}
+
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+
+inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
+    int64_t g = (count + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+// Stable CSR -> CSC on the device: radix-sort the CSR positions by column (stable, so rows
+// stay ascending inside every column -- the reference's counting-sort order), then gather.
+int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, const double* val,
+                     int* colptr, int* rowidx, double* cval, hipStream_t s) {
+    DevBuf row_of, pos_in, pos_out, keys_out, tmp;
+    SPT_HIP(hipMalloc(&row_of.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
+    SPT_HIP(hipMalloc(&pos_in.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
+    SPT_HIP(hipMalloc(&pos_out.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
+    SPT_HIP(hipMalloc(&keys_out.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
+    k_expand_rows<<<grid_for(n), 256, 0, s>>>(rowptr, n, (int*)row_of.p);
+    k_iota<<<grid_for(nnz), 256, 0, s>>>((int*)pos_in.p, nnz);
+    int end_bit = 1;
+    while (end_bit < 31 && (1 << end_bit) <= n) ++end_bit;
+    size_t tmp_bytes = 0, need = 0;
+    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, col, (int*)keys_out.p, (int*)pos_in.p,
+                                               (int*)pos_out.p, (int)nnz, 0, end_bit, s));
+    tmp_bytes = need;
+    SPT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, colptr, colptr, n + 1, s));
+    tmp_bytes = std::max(tmp_bytes, need);
+    SPT_HIP(hipMalloc(&tmp.p, tmp_bytes));
+    need = tmp_bytes;
+    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, col, (int*)keys_out.p, (int*)pos_in.p,
+                                               (int*)pos_out.p, (int)nnz, 0, end_bit, s));
+    k_gather_csc<<<grid_for(nnz), 256, 0, s>>>((int*)pos_out.p, (int*)row_of.p, val, nnz, rowidx, cval);
+    // colptr: histogram of columns (shifted by one) then inclusive scan
+    k_fill_int<<<grid_for(n + 1), 256, 0, s>>>(colptr, 0, n + 1);
+    k_histogram<<<grid_for(nnz), 256, 0, s>>>(col, nnz, 1, colptr);
+    need = tmp_bytes;
+    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, colptr, colptr, n + 1, s));
+    SPT_HIP(hipStreamSynchronize(s));
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int analyse(sptrsv_plan_s* p) {
+    const int n = p->n;
+    const int64_t nnz = p->nnz;
+    hipStream_t s = p->stream;
+
+    DevBuf tmp, keys_out, pos_in;
+    size_t tmp_bytes = 0, need = 0;
+    SPT_CHECK(device_transpose(n, nnz, p->rowptr, p->col, p->val, p->colptr, p->rowidx, p->cval, s));
+    SPT_HIP(hipMalloc(&keys_out.p, (size_t)n * sizeof(int)));
+    SPT_HIP(hipMalloc(&pos_in.p, (size_t)n * sizeof(int)));
+    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, p->levels, (int*)keys_out.p,
+                                               (int*)pos_in.p, p->level_rows, n, 0, 31, s));
+    tmp_bytes = need;
+    SPT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, p->lrowptr, p->lrowptr, n + 1, s));
+    tmp_bytes = std::max(tmp_bytes, need);
+    SPT_HIP(hipMalloc(&tmp.p, tmp_bytes));
+
+    // ---- in-degree: the reference analyser's histogram over CSC row indices ---------------
+    k_fill_int<<<grid_for(n), 256, 0, s>>>(p->indeg, 0, n);
+    k_histogram<<<grid_for(nnz), 256, 0, s>>>(p->rowidx, nnz, 0, p->indeg);
+
+    // ---- levels --------------------------------------------------------------------------
+    k_fill_int<<<grid_for(n), 256, 0, s>>>(p->levels, 0, n);
+    k_fill_int<<<1, 64, 0, s>>>(p->ticket, 0, 64);
+    const int waves = (n + kWave - 1) / kWave;
+    const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
+    k_levels<<<(waves + 3) / 4, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
+    int ctrl[3] = {0, 0, 0};
+    SPT_HIP(hipMemcpyAsync(ctrl, p->ticket, sizeof(ctrl), hipMemcpyDeviceToHost, s));
+    SPT_HIP(hipStreamSynchronize(s));
+    if (ctrl[1] != 0)
+        return fail(SPTRSV_ERR_DEADLOCK, "level analysis watchdog fired");
+    const int L = ctrl[2];
+    p->n_levels = L;
+
+    // ---- level pointers: histogram of levels + inclusive scan (level_ptr[0] = 0) -------------
+    SPT_HIP(hipMalloc((void**)&p->level_ptr, ((size_t)L + 1) * sizeof(int)));
+    k_fill_int<<<grid_for(L + 1), 256, 0, s>>>(p->level_ptr, 0, L + 1);
+    k_histogram<<<grid_for(n), 256, 0, s>>>(p->levels, n, 0, p->level_ptr);
+    need = tmp_bytes;
+    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, p->level_ptr, p->level_ptr, L + 1, s));
+
+    // ---- level sets: stable sort of row ids by level (rows stay ascending per level) ------
+    int lbits = 1;
+    while (lbits < 31 && (1 << lbits) <= L) ++lbits;
+    k_iota<<<grid_for(n), 256, 0, s>>>((int*)pos_in.p, n);
+    need = tmp_bytes;
+    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, p->levels, (int*)keys_out.p,
+                                               (int*)pos_in.p, p->level_rows, n, 0, lbits, s));
+
+    // ---- level-ordered CSR ---------------------------------------------------------------
+    k_level_rowlen<<<grid_for(n), 256, 0, s>>>(p->rowptr, p->level_rows, n, p->lrowptr);
+    need = tmp_bytes;
+    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, p->lrowptr, p->lrowptr, n + 1, s));
+    k_level_gather<<<grid_for((int64_t)n * kWave), 256, 0, s>>>(p->rowptr, p->col, p->val, p->level_rows,
+                                                                p->lrowptr, n, p->lcol, p->lval);
+    p->h_level_ptr.resize((size_t)L + 1);
+    SPT_HIP(hipMemcpyAsync(p->h_level_ptr.data(), p->level_ptr, ((size_t)L + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    SPT_HIP(hipStreamSynchronize(s));
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int ensure_fp32(sptrsv_plan_s* p) {
+    if (p->val32) return SPTRSV_OK;
+    const int64_t nnz = p->nnz;
+    SPT_HIP(hipMalloc((void**)&p->val32, nnz * sizeof(float)));
+    SPT_HIP(hipMalloc((void**)&p->cval32, nnz * sizeof(float)));
+    SPT_HIP(hipMalloc((void**)&p->lval32, nnz * sizeof(float)));
+    k_to_float<<<grid_for(nnz), 256, 0, p->stream>>>(p->val, p->val32, nnz);
+    k_to_float<<<grid_for(nnz), 256, 0, p->stream>>>(p->cval, p->cval32, nnz);
+    k_to_float<<<grid_for(nnz), 256, 0, p->stream>>>(p->lval, p->lval32, nnz);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+}  // namespace sptrsv
+
+extern "C" int sptrsv_csr_to_csc(int n, int64_t nnz, const int64_t* rowptr, const int* col, const double* val,
+                                 int* colptr, int* rowidx, double* cval) {
+    using namespace sptrsv;
+    if (n <= 0 || nnz < 0 || !rowptr || !col || !val || !colptr || !rowidx || !cval)
+        return fail(SPTRSV_ERR_BAD_INPUT, "bad transpose arguments");
+    if (nnz >= ((int64_t)1 << 31)) return fail(SPTRSV_ERR_UNSUPPORTED, "nnz >= 2^31");
+    if (rowptr[0] != 0 || rowptr[n] != nnz) return fail(SPTRSV_ERR_BAD_INPUT, "rowptr does not span nnz");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (col[k] < 0 || col[k] >= n) return fail(SPTRSV_ERR_BAD_INPUT, "column index out of range");
+    for (int i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return fail(SPTRSV_ERR_BAD_INPUT, "rowptr not monotone");
+    SPT_CHECK(ensure_device());
+    std::vector<int> rp((size_t)n + 1);
+    for (int i = 0; i <= n; ++i) rp[i] = (int)rowptr[i];
+    DevBuf d_rp, d_col, d_val, d_cp, d_ri, d_cv;
+    const size_t z = (size_t)std::max<int64_t>(nnz, 1);
+    SPT_HIP(hipMalloc(&d_rp.p, rp.size() * sizeof(int)));
+    SPT_HIP(hipMalloc(&d_col.p, z * sizeof(int)));
+    SPT_HIP(hipMalloc(&d_val.p, z * sizeof(double)));
+    SPT_HIP(hipMalloc(&d_cp.p, ((size_t)n + 1) * sizeof(int)));
+    SPT_HIP(hipMalloc(&d_ri.p, z * sizeof(int)));
+    SPT_HIP(hipMalloc(&d_cv.p, z * sizeof(double)));
+    SPT_HIP(hipMemcpy(d_rp.p, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice));
+    SPT_HIP(hipMemcpy(d_col.p, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+    SPT_HIP(hipMemcpy(d_val.p, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
+    SPT_CHECK(device_transpose(n, nnz, (int*)d_rp.p, (int*)d_col.p, (double*)d_val.p, (int*)d_cp.p,
+                               (int*)d_ri.p, (double*)d_cv.p, nullptr));
+    SPT_HIP(hipMemcpy(colptr, d_cp.p, ((size_t)n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(rowidx, d_ri.p, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(cval, d_cv.p, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost));
+    return SPTRSV_OK;
+}

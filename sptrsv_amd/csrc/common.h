@@ -1,0 +1,102 @@
+// common.h -- shared host/device helpers of the gfx950 solver library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "sptrsv_capi.h"
+
+// ---- host-side error plumbing ------------------------------------------------------------
+namespace sptrsv {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define SPT_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::sptrsv::fail(SPTRSV_ERR_HIP, std::string(#expr " -> ") +          \
+                                                      hipGetErrorString(e_));          \
+    } while (0)
+
+#define SPT_CHECK(expr)            \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != SPTRSV_OK)      \
+            return rc_;            \
+    } while (0)
+
+int ensure_device();
+uint64_t wallclock_ticks_per_ms();
+
+// ---- device-side helpers -------------------------------------------------------------------
+constexpr int kWave = 64;
+
+// Values of x that are still unknown hold a signalling-NaN bit pattern.  IEEE arithmetic
+// never produces a signalling NaN (invalid operations give the default quiet NaN), so a
+// computed x can never be mistaken for "not ready", and the value itself is the ready flag.
+constexpr uint64_t kSentinel64 = 0x7FF4DEADBEEF5AA5ull;
+constexpr uint32_t kSentinel32 = 0x7FA5DEADu;
+
+template <typename T> struct Bits;
+template <> struct Bits<double> {
+    using U = uint64_t;
+    static constexpr U sentinel = kSentinel64;
+};
+template <> struct Bits<float> {
+    using U = uint32_t;
+    static constexpr U sentinel = kSentinel32;
+};
+
+// Relaxed agent-scope load/store: `global_load/store ... sc1` on gfx950 -- coherent across
+// the XCDs' L2s without the L2 write-back/invalidate that acquire/release would add.
+template <typename T>
+__device__ __forceinline__ typename Bits<T>::U poll_bits(const T* p) {
+    using U = typename Bits<T>::U;
+    return __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v) {
+    using U = typename Bits<T>::U;
+    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T from_bits(typename Bits<T>::U u) {
+    return __builtin_bit_cast(T, u);
+}
+
+// Wave-level work ticket: lane 0 takes the next ticket, every lane gets it.  Tickets are
+// handed out in launch order of the waves, so a wave only ever waits on work whose ticket
+// is smaller -- held by a wave that is already resident -- which guarantees progress.
+__device__ __forceinline__ int wave_ticket(int* counter) {
+    int t = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(counter, 1);
+    return __shfl(t, 0, kWave);
+}
+
+// Watchdog: a lane that has waited longer than `limit` wall-clock ticks sets the status
+// word and gives up; every waiting lane also gives up once the status word is non-zero,
+// so a malformed input ends the kernel instead of hanging the device.
+struct Watchdog {
+    uint64_t start;
+    uint64_t limit;
+    int* status;
+    __device__ __forceinline__ void arm(int* s, uint64_t lim) {
+        status = s;
+        limit = lim;
+        start = wall_clock64();
+    }
+    __device__ __forceinline__ bool expired() {
+        if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        if (wall_clock64() - start > limit) {
+            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        return false;
+    }
+};
+
+}  // namespace sptrsv

@@ -1,0 +1,564 @@
+// solve.hip -- gfx950 sync-free / level-set triangular solve kernels for U X = B.
+//
+// Families (SPTRSV_VAR_*):
+//   ROW_PULL / LEVEL_PULL  k_pull: G lanes per CSR row (lanes split the RHS columns),
+//       rows handed out by wave tickets in descending order (ROW) or level order (LEVEL,
+//       over the level-ordered CSR).  A lane waits for each needed x value by polling it:
+//       unknown values hold a signalling-NaN sentinel, so x is published by one relaxed
+//       agent-scope store with no separate flag and no L2 write-back.  Products are summed
+//       in ascending column order without FMA contraction and divided by the diagonal --
+//       the reference serial solver's arithmetic (src/core.cpp:18-32), so FP64 results are
+//       bitwise equal to it.
+//   ROW_PULL with m == 1 and group > 1  k_pull_split: G lanes split the ENTRIES of a row
+//       (long rows), partial sums reduced across the group with cross-lane shuffles.
+//   CSC_PUSH  k_push: one wave per CSC column, the reference's algorithm
+//       (src/kernel_full.cl:54-228): wait for in-degree == 1, x = (b - left_sum) / diag,
+//       scatter x * U(:,j) into left_sum with native FP64 atomics, release, decrement the
+//       in-degree of every touched row.
+//   LEVELSET  k_level_step: bulk-synchronous, one launch per level over the level-ordered
+//       CSR (src/core.cpp:273-376 without the OpenMP task split); plain loads, exact order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "common.h"
+#include "plan.h"
+
+namespace sptrsv {
+namespace {
+
+template <typename T>
+struct PullArgs {
+    int n;             // rows
+    int m;             // RHS count (row stride of b and x)
+    int k0;            // first RHS column handled by this launch
+    int kc;            // number of RHS columns handled by this launch
+    const int* rowptr; // CSR (natural order or level order)
+    const int* col;
+    const T* val;
+    const int* perm;   // level order: CSR row k is original row perm[k]; null otherwise
+    const T* b;
+    T* x;
+    int* ctrl;         // [0] ticket, [1] status
+    uint64_t limit;    // watchdog ticks
+};
+
+template <typename T, int G, int KPL, int UNROLL, bool LEVELED>
+__global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
+#pragma clang fp contract(off)
+    using U = typename Bits<T>::U;
+    constexpr int RPW = kWave / G;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = wave_ticket(&a.ctrl[0]);
+    const int slot = t * RPW + lane / G;
+    const int gl = lane % G;
+    bool done = slot >= a.n || gl >= a.kc;
+    int row_csr = 0, row_out = 0, p = 0, end = 0;
+    T diag = T(1);
+    T acc[KPL];
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) acc[j] = T(0);
+    if (!done) {
+        row_csr = LEVELED ? slot : a.n - 1 - slot;
+        row_out = LEVELED ? a.perm[slot] : row_csr;
+        const int s = a.rowptr[row_csr];
+        diag = a.val[s];
+        p = s + 1;
+        end = a.rowptr[row_csr + 1];
+    }
+    const size_t m = (size_t)a.m;
+    Watchdog wd;
+    wd.arm(&a.ctrl[1], a.limit);
+    int idle = 0;
+    while (true) {
+        bool progressed = false;
+        if (!done) {
+            while (p < end) {
+                const int cnt = min(UNROLL, end - p);
+                int c[UNROLL];
+                T v[UNROLL];
+                U xb[UNROLL][KPL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    if (u < cnt) {
+                        c[u] = a.col[p + u];
+                        v[u] = a.val[p + u];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+#pragma unroll
+                    for (int j = 0; j < KPL; ++j) {
+                        const int k = gl + j * G;
+                        xb[u][j] = Bits<T>::sentinel;
+                        if (u < cnt && k < a.kc) xb[u][j] = poll_bits(a.x + (size_t)c[u] * m + a.k0 + k);
+                    }
+                }
+                int ok = 0;
+                bool stop = false;
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    bool ready = u < cnt && !stop;
+#pragma unroll
+                    for (int j = 0; j < KPL; ++j)
+                        if (gl + j * G < a.kc && xb[u][j] == Bits<T>::sentinel) ready = false;
+                    if (ready) {
+#pragma unroll
+                        for (int j = 0; j < KPL; ++j) acc[j] = acc[j] + v[u] * from_bits<T>(xb[u][j]);
+                        ++ok;
+                    } else {
+                        stop = true;
+                    }
+                }
+                p += ok;
+                if (ok) progressed = true;
+                if (ok < cnt) break;
+            }
+            if (p >= end) {
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) {
+                    const int k = gl + j * G;
+                    if (k < a.kc) {
+                        const size_t at = (size_t)row_out * m + a.k0 + k;
+                        publish(a.x + at, (a.b[at] - acc[j]) / diag);
+                    }
+                }
+                done = true;
+                progressed = true;
+            }
+        }
+        if (__all(done)) break;
+        if (!__any(progressed)) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
+        }
+    }
+}
+
+// m == 1, G lanes per row splitting the row's entries (lane e of a group takes entries
+// e, e+G, ...).  Rows of one wave can depend on each other, so a group publishes from inside
+// the polling loop: every iteration all lanes reduce their partial sums across the group
+// (cross-lane shuffles need every lane active), and a group whose lanes have all finished
+// their share publishes once.
+template <typename T, int G, bool LEVELED>
+__global__ __launch_bounds__(256) void k_pull_split(PullArgs<T> a) {
+    using U = typename Bits<T>::U;
+    constexpr int RPW = kWave / G;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = wave_ticket(&a.ctrl[0]);
+    const int slot = t * RPW + lane / G;
+    const int gl = lane % G;
+    const bool has_row = slot < a.n;
+    bool share_done = !has_row, published = !has_row;
+    int row_out = 0, p = 0, end = 0;
+    T diag = T(1), acc = T(0);
+    if (has_row) {
+        const int row_csr = LEVELED ? slot : a.n - 1 - slot;
+        row_out = LEVELED ? a.perm[slot] : row_csr;
+        const int s = a.rowptr[row_csr];
+        diag = a.val[s];
+        p = s + 1 + gl;
+        end = a.rowptr[row_csr + 1];
+    }
+    const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << ((lane / G) * G));
+    Watchdog wd;
+    wd.arm(&a.ctrl[1], a.limit);
+    int idle = 0;
+    while (true) {
+        bool progressed = false;
+        if (!share_done) {
+            while (p < end) {
+                const U xb = poll_bits(a.x + a.col[p]);
+                if (xb == Bits<T>::sentinel) break;
+                acc += a.val[p] * from_bits<T>(xb);
+                p += G;
+                progressed = true;
+            }
+            if (p >= end) share_done = true;
+        }
+        const bool group_done = (__ballot(share_done) & gmask) == gmask;
+        T tot = acc;
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, G);
+        if (group_done && !published) {
+            if (gl == 0) publish(a.x + row_out, (a.b[row_out] - tot) / diag);
+            published = true;
+            progressed = true;
+        }
+        if (__all(published)) break;
+        if (!__any(progressed)) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && !share_done && wd.expired()) share_done = true;
+        }
+    }
+}
+
+template <typename T>
+struct PushArgs {
+    int n, m;
+    const int* colptr;
+    const int* rowidx;
+    const T* val;
+    int* indeg;      // consumed
+    T* left_sum;     // n*m, zeroed
+    const T* b;
+    T* x;
+    int* ctrl;
+    uint64_t limit;
+};
+
+// One wave per CSC column, descending by ticket (the reference: src/kernel_full.cl:54-228).
+template <typename T>
+__global__ __launch_bounds__(64) void k_push(PushArgs<T> a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = wave_ticket(&a.ctrl[0]);
+    const int j = a.n - 1 - t;
+    if (j < 0) return;
+    const int cs = a.colptr[j], ce = a.colptr[j + 1] - 1;
+    const T diag = a.val[ce];
+    if (lane == 0) {
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        int idle = 0;
+        while (__hip_atomic_load(&a.indeg[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 1) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && wd.expired()) break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const size_t m = (size_t)a.m;
+    if (a.m == 1) {
+        T xj = T(0);
+        if (lane == 0) {
+            xj = (a.b[j] - a.left_sum[j]) / diag;
+            a.x[j] = xj;
+        }
+        xj = __shfl(xj, 0, kWave);
+        for (int e = cs + lane; e < ce; e += kWave)
+            atomicAdd(&a.left_sum[a.rowidx[e]], a.val[e] * xj);
+    } else {
+        for (int k = lane; k < a.m; k += kWave) {
+            const size_t at = (size_t)j * m + k;
+            a.x[at] = (a.b[at] - a.left_sum[at]) / diag;
+        }
+        for (int e = cs; e < ce; ++e) {
+            const int r = a.rowidx[e];
+            const T v = a.val[e];
+            for (int k = lane; k < a.m; k += kWave)
+                atomicAdd(&a.left_sum[(size_t)r * m + k], v * a.x[(size_t)j * m + k]);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int e = cs + lane; e < ce; e += kWave)
+        __hip_atomic_fetch_add(&a.indeg[a.rowidx[e]], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bulk-synchronous level step: rows [lo, hi) of the level-ordered CSR, G lanes per row over
+// the RHS columns.  Every dependency lies in an earlier level, i.e. an earlier launch.
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_level_step(int lo, int hi, int m, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const T* __restrict__ val,
+                                                    const int* __restrict__ perm, const T* __restrict__ b,
+                                                    T* __restrict__ x) {
+#pragma clang fp contract(off)
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
+    const int slot = lo + gid / G;
+    const int gl = gid % G;
+    if (slot >= hi) return;
+    const int r = perm[slot];
+    const int s = rowptr[slot], e = rowptr[slot + 1];
+    const T diag = val[s];
+    for (int k = gl; k < m; k += G) {
+        T acc = T(0);
+        for (int q = s + 1; q < e; ++q) acc = acc + val[q] * x[(size_t)col[q] * m + k];
+        const size_t at = (size_t)r * m + k;
+        x[at] = (b[at] - acc) / diag;
+    }
+}
+
+template <typename T>
+__global__ void k_fill_bits(T* out, typename Bits<T>::U bits, int64_t count) {
+    using U = typename Bits<T>::U;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) reinterpret_cast<U*>(out)[i] = bits;
+}
+
+__global__ void k_copy_int(const int* __restrict__ in, int* __restrict__ out, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
+__global__ void k_reset_ticket(int* ctrl) {
+    if (threadIdx.x == 0) ctrl[0] = 0;
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_convert(const Tin* __restrict__ in, Tout* __restrict__ out, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = (Tout)in[i];
+}
+
+// r = b - U x, FP64, G lanes per row over the RHS columns; products in ascending order.
+__global__ __launch_bounds__(256) void k_residual(int n, int m, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ x, const double* __restrict__ b,
+                                                  double* __restrict__ r) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)n * m;
+    for (int64_t q = gid; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int row = (int)(q / m), k = (int)(q % m);
+        double acc = 0.0;
+        for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
+        r[q] = b[q] - acc;
+    }
+}
+
+inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
+    int64_t g = (count + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+inline int next_pow2(int v) {
+    int g = 1;
+    while (g < v) g <<= 1;
+    return g;
+}
+
+template <typename T, bool LEVELED>
+int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int unroll) {
+    // lanes per row: m == 1 -> 1 (or `group` lanes splitting entries); m <= 64 -> next pow2 of
+    // m; m > 64 -> 64 lanes with up to 4 columns each, launched in chunks of 256 columns.
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    if (m == 1 && group > 1) {
+        const int G = next_pow2(std::min(group, 64));
+        const int waves = (int)(((int64_t)n * G + kWave - 1) / kWave);
+        PullArgs<T> a = base;
+        a.k0 = 0;
+        a.kc = 1;
+        hipEvent_t stop;
+        SPT_CHECK(begin_timed(P, &stop));
+        k_reset_ticket<<<1, 64, 0, s>>>(a.ctrl);
+        const dim3 grid((waves + 3) / 4);
+        switch (G) {
+            case 2: k_pull_split<T, 2, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            case 4: k_pull_split<T, 4, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            case 8: k_pull_split<T, 8, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            case 16: k_pull_split<T, 16, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            case 32: k_pull_split<T, 32, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            default: k_pull_split<T, 64, LEVELED><<<grid, 256, 0, s>>>(a); break;
+        }
+        end_timed(P, stop);
+        SPT_HIP(hipGetLastError());
+        return SPTRSV_OK;
+    }
+    for (int k0 = 0; k0 < m; k0 += 256) {
+        const int kc = std::min(256, m - k0);
+        PullArgs<T> a = base;
+        a.k0 = k0;
+        a.kc = kc;
+        const int G = kc == 1 ? 1 : std::min(64, next_pow2(kc));
+        const int kpl = (kc + G - 1) / G;  // 1..4
+        const int rows_per_wave = kWave / G;
+        const int waves = (n + rows_per_wave - 1) / rows_per_wave;
+        const dim3 grid((waves + 3) / 4);
+        hipEvent_t stop;
+        SPT_CHECK(begin_timed(P, &stop));
+        k_reset_ticket<<<1, 64, 0, s>>>(a.ctrl);
+#define SPT_PULL(G_, K_, U_) k_pull<T, G_, K_, U_, LEVELED><<<grid, 256, 0, s>>>(a)
+        if (G == 1) {
+            if (unroll == 1) SPT_PULL(1, 1, 1);
+            else if (unroll == 2) SPT_PULL(1, 1, 2);
+            else SPT_PULL(1, 1, 4);
+        } else if (G == 2) SPT_PULL(2, 1, 2);
+        else if (G == 4) SPT_PULL(4, 1, 2);
+        else if (G == 8) SPT_PULL(8, 1, 2);
+        else if (G == 16) SPT_PULL(16, 1, 2);
+        else if (G == 32) SPT_PULL(32, 1, 2);
+        else if (kpl == 1) SPT_PULL(64, 1, 2);
+        else if (kpl == 2) SPT_PULL(64, 2, 1);
+        else SPT_PULL(64, 4, 1);
+#undef SPT_PULL
+        end_timed(P, stop);
+        SPT_HIP(hipGetLastError());
+    }
+    return SPTRSV_OK;
+}
+
+template <typename T>
+int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, const T* d_b, T* d_x, int m,
+                const T* val, const T* cval, const T* lval, uint64_t limit) {
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    const int64_t nm = (int64_t)n * m;
+    switch (variant) {
+        case SPTRSV_VAR_ROW_PULL:
+        case SPTRSV_VAR_LEVEL_PULL: {
+            k_fill_bits<T><<<grid_for(nm), 256, 0, s>>>(d_x, Bits<T>::sentinel, nm);
+            PullArgs<T> a{};
+            a.n = n;
+            a.m = m;
+            a.b = d_b;
+            a.x = d_x;
+            a.ctrl = P->ticket;
+            a.limit = limit;
+            if (variant == SPTRSV_VAR_ROW_PULL) {
+                a.rowptr = P->rowptr;
+                a.col = P->col;
+                a.val = val;
+                a.perm = nullptr;
+                return launch_pull<T, false>(P, a, m, group, unroll);
+            }
+            a.rowptr = P->lrowptr;
+            a.col = P->lcol;
+            a.val = lval;
+            a.perm = P->level_rows;
+            return launch_pull<T, true>(P, a, m, group, unroll);
+        }
+        case SPTRSV_VAR_CSC_PUSH: {
+            const size_t need = (size_t)nm * sizeof(T);
+            if (P->left_sum_elems * sizeof(double) < need) {
+                if (P->left_sum) SPT_HIP(hipFree(P->left_sum));
+                P->left_sum = nullptr;
+                SPT_HIP(hipMalloc((void**)&P->left_sum, need));
+                P->left_sum_elems = (need + sizeof(double) - 1) / sizeof(double);
+            }
+            T* ls = reinterpret_cast<T*>(P->left_sum);
+            k_fill_bits<T><<<grid_for(nm), 256, 0, s>>>(ls, 0, nm);
+            k_copy_int<<<grid_for(n), 256, 0, s>>>(P->indeg, P->indeg_work, n);
+            k_reset_ticket<<<1, 64, 0, s>>>(P->ticket);
+            PushArgs<T> a{n, m, P->colptr, P->rowidx, cval, P->indeg_work, ls, d_b, d_x, P->ticket, limit};
+            hipEvent_t stop;
+            SPT_CHECK(begin_timed(P, &stop));
+            k_push<T><<<n, 64, 0, s>>>(a);
+            end_timed(P, stop);
+            SPT_HIP(hipGetLastError());
+            return SPTRSV_OK;
+        }
+        case SPTRSV_VAR_LEVELSET: {
+            const int G = m == 1 ? 1 : std::min(64, next_pow2(m));
+            hipEvent_t stop;
+            SPT_CHECK(begin_timed(P, &stop));
+            for (int l = 0; l < P->n_levels; ++l) {
+                const int lo = P->h_level_ptr[l], hi = P->h_level_ptr[l + 1];
+                const int64_t threads = (int64_t)(hi - lo) * G;
+                const int grid = (int)((threads + 255) / 256);
+                switch (G) {
+                    case 1: k_level_step<T, 1><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    case 2: k_level_step<T, 2><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    case 4: k_level_step<T, 4><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    case 8: k_level_step<T, 8><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    case 16: k_level_step<T, 16><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    case 32: k_level_step<T, 32><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                    default: k_level_step<T, 64><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
+                }
+            }
+            end_timed(P, stop);
+            SPT_HIP(hipGetLastError());
+            return SPTRSV_OK;
+        }
+        default:
+            return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
+    }
+}
+
+}  // namespace
+
+int default_variant(const sptrsv_plan_s* P, int m) {
+    (void)m;
+    (void)P;
+    return SPTRSV_VAR_LEVEL_PULL;
+}
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" {
+
+int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d_b, double* d_x, int m) {
+    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    sptrsv_params_t prm{};
+    if (params) prm = *params;
+    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
+    if (variant < 1 || variant >= SPTRSV_VAR_COUNT || variant == SPTRSV_VAR_WINDOW)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
+    const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
+                                            : P->ticks_per_ms * 20000ull;
+    const int unroll = prm.unroll > 0 ? prm.unroll : 4;
+    if (prm.precision == SPTRSV_FP64)
+        return solve_typed<double>(P, variant, prm.group, unroll, d_b, d_x, m, P->val, P->cval, P->lval, limit);
+    if (prm.precision == SPTRSV_FP32) {
+        SPT_CHECK(ensure_fp32(P));
+        const int64_t nm = (int64_t)P->n * m;
+        if ((int64_t)P->stage_elems < nm) {
+            if (P->b32) SPT_HIP(hipFree(P->b32));
+            if (P->x32) SPT_HIP(hipFree(P->x32));
+            P->b32 = P->x32 = nullptr;
+            SPT_HIP(hipMalloc((void**)&P->b32, nm * sizeof(float)));
+            SPT_HIP(hipMalloc((void**)&P->x32, nm * sizeof(float)));
+            P->stage_elems = (size_t)nm;
+        }
+        k_convert<double, float><<<grid_for(nm), 256, 0, P->stream>>>(d_b, P->b32, nm);
+        SPT_CHECK(solve_typed<float>(P, variant, prm.group, unroll, P->b32, P->x32, m, P->val32, P->cval32,
+                                     P->lval32, limit));
+        k_convert<float, double><<<grid_for(nm), 256, 0, P->stream>>>(P->x32, d_x, nm);
+        SPT_HIP(hipGetLastError());
+        return SPTRSV_OK;
+    }
+    return fail(SPTRSV_ERR_BAD_INPUT, "unknown precision");
+}
+
+int sptrsv_solve_host(sptrsv_plan_t P, const sptrsv_params_t* params, const double* b, double* x, int m,
+                      int col_major) {
+    if (!P || !b || !x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    const size_t n = (size_t)P->n, count = n * (size_t)m;
+    std::string err;
+    double *db = nullptr, *dx = nullptr;
+    SPT_HIP(hipMalloc((void**)&db, count * sizeof(double)));
+    if (hipMalloc((void**)&dx, count * sizeof(double)) != hipSuccess) {
+        (void)hipFree(db);
+        return fail(SPTRSV_ERR_NOMEM, "device allocation of x failed");
+    }
+    int rc = SPTRSV_OK;
+    if (col_major && m > 1) {
+        // column-major host <-> row-major device
+        std::vector<double> tmp(count);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < m; ++k) tmp[i * m + k] = b[i + k * n];
+        if (hipMemcpy(db, tmp.data(), count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(SPTRSV_ERR_HIP, "upload of b failed");
+        if (rc == SPTRSV_OK) rc = sptrsv_solve(P, params, db, dx, m);
+        if (rc == SPTRSV_OK) rc = sptrsv_plan_sync(P);
+        if (rc == SPTRSV_OK && hipMemcpy(tmp.data(), dx, count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(SPTRSV_ERR_HIP, "download of x failed");
+        if (rc == SPTRSV_OK)
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < m; ++k) x[i + k * n] = tmp[i * m + k];
+    } else {
+        if (hipMemcpy(db, b, count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(SPTRSV_ERR_HIP, "upload of b failed");
+        if (rc == SPTRSV_OK) rc = sptrsv_solve(P, params, db, dx, m);
+        if (rc == SPTRSV_OK) rc = sptrsv_plan_sync(P);
+        if (rc == SPTRSV_OK && hipMemcpy(x, dx, count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(SPTRSV_ERR_HIP, "download of x failed");
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return rc;
+}
+
+int sptrsv_residual(sptrsv_plan_t P, const double* d_b, const double* d_x, double* d_r, int m) {
+    if (!P || !d_b || !d_x || !d_r) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    const int64_t nm = (int64_t)P->n * m;
+    k_residual<<<grid_for(nm), 256, 0, P->stream>>>(P->n, m, P->rowptr, P->col, P->val, d_x, d_b, d_r);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+}  // extern "C"

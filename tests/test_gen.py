@@ -1,0 +1,131 @@
+"""Synthetic workload generators: library cross-checks and structural invariants."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from sptrsv_amd import gen
+
+
+def laplacian2d(N):
+    T = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(N, N))
+    A = (sp.kron(sp.identity(N), T) + sp.kron(T, sp.identity(N))).tocsr()
+    A.eliminate_zeros()
+    return A
+
+
+def laplacian3d(N):
+    T = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(N, N))
+    I = sp.identity(N)
+    A = (sp.kron(sp.kron(I, I), T) + sp.kron(sp.kron(I, T), I) + sp.kron(sp.kron(T, I), I)).tocsr()
+    A.eliminate_zeros()
+    return A
+
+
+def is_upper_diag_first(U):
+    rows = np.repeat(np.arange(U.n), np.diff(U.rowptr))
+    first = U.rowptr[:-1]
+    ok = np.all(U.col[first] == np.arange(U.n))
+    off = np.ones(U.nnz, bool)
+    off[first] = False
+    ok &= np.all(U.col[off] > rows[off])
+    # strictly ascending within rows
+    d = np.diff(U.col.astype(np.int64))
+    same_row = rows[1:] == rows[:-1]
+    return bool(ok and np.all(d[same_row] > 0))
+
+
+@pytest.mark.parametrize("N", [10, 100])
+def test_laplace2d_cholesky_matches_scipy(N):
+    U = gen.laplace2d_cholesky(N)
+    n = N * N
+    assert U.n == n and U.nnz == sum(min(N + 1, n - i) for i in range(n))
+    assert is_upper_diag_first(U)
+    A = laplacian2d(N)
+    ab = np.zeros((N + 1, n))
+    for d in range(N + 1):
+        ab[N - d, d:] = A.diagonal(d)
+    cb = sla.cholesky_banded(ab, lower=False)
+    rows = np.repeat(np.arange(n), np.diff(U.rowptr))
+    np.testing.assert_allclose(U.val, cb[N + rows - U.col, U.col], rtol=1e-13, atol=1e-14)
+
+
+def test_laplace2d_full_size_counts():
+    U = gen.laplace2d_cholesky(100)
+    assert (U.n, U.nnz) == (10_000, 1_004_950)
+
+
+@pytest.mark.parametrize("N", [5, 12])
+def test_ic0_reproduces_A_on_its_pattern(N):
+    U = gen.laplace3d_ic0(N)
+    assert is_upper_diag_first(U)
+    A = laplacian3d(N)
+    S = U.to_scipy()
+    P = (S.T @ S).tocsr()
+    Au = sp.triu(A).tocsr()
+    Au.sort_indices()
+    rows = np.repeat(np.arange(U.n), np.diff(Au.indptr))
+    np.testing.assert_allclose(np.asarray(P[rows, Au.indices]).ravel(), Au.data, rtol=1e-13, atol=1e-13)
+    # same pattern as triu(A)
+    assert np.array_equal(S.indptr, Au.indptr) and np.array_equal(S.indices, Au.indices)
+
+
+def test_ic0_full_size_counts():
+    U = gen.laplace3d_ic0(128)
+    assert (U.n, U.nnz) == (2_097_152, 8_339_456)
+    lv, L = O.levels(U.rowptr, U.col)
+    assert L == 382
+
+
+def test_random_zipf_structure_and_determinism():
+    U = gen.random_zipf(20_000, seed=1)
+    assert is_upper_diag_first(U)
+    V = gen.random_zipf(20_000, seed=1)
+    assert np.array_equal(U.col, V.col) and np.array_equal(U.val, V.val)
+    first = U.rowptr[:-1]
+    rows = np.repeat(np.arange(U.n), np.diff(U.rowptr))
+    off = np.ones(U.nnz, bool)
+    off[first] = False
+    absum = np.bincount(rows[off], weights=np.abs(U.val[off]), minlength=U.n)
+    np.testing.assert_allclose(U.val[first], 1.0 + absum, rtol=1e-15)
+    assert np.all(np.abs(U.val[off]) < 1.0)
+    # mostly a chain: row i depends on i+1 for most rows (Zipf(1.5) puts ~38% mass on offset 1)
+    has_next = np.zeros(U.n, bool)
+    has_next[rows[off][U.col[off] == rows[off] + 1]] = True
+    assert has_next.mean() > 0.9
+
+
+def test_random_supernodal_structure():
+    U = gen.random_supernodal(40, seed=2)
+    assert is_upper_diag_first(U)
+    sn = U.snodes
+    assert sn[0] == 0 and sn[-1] == U.n and np.all(np.diff(sn) >= 16) and np.all(np.diff(sn) <= 128)
+    for k in range(len(sn) - 1):
+        s, e = sn[k], sn[k + 1]
+        w = e - s
+        tails = []
+        for d in range(w):
+            r = s + d
+            cols = U.col[U.rowptr[r]:U.rowptr[r + 1]]
+            assert np.array_equal(cols[:w - d], np.arange(r, e))  # dense triangle row
+            tails.append(cols[w - d:])
+        assert all(np.array_equal(t, tails[0]) for t in tails)  # identical rectangular pattern
+        assert np.all(tails[0] >= e)
+
+
+def test_rhs_layouts():
+    U = gen.random_zipf(500, seed=3)
+    xt, b = gen.rhs_for(U, 3, seed=0, layout="row")
+    assert xt.shape == (500, 3) and b.shape == (500, 3)
+    xc, bc = gen.rhs_for(U, 3, seed=0, layout="col")
+    assert np.array_equal(xc, xt.T) and np.allclose(bc, b.T)
+    x, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, bc)
+    assert O.rel_error(xc, x) < 1e-12
+
+
+def test_to_csc_matches_oracle_transpose():
+    U = gen.random_zipf(800, seed=4)
+    cp, ri, cv = gen.to_csc(U)
+    ocp, ori, ocv = O.transpose(U.rowptr, U.col, U.val)
+    assert np.array_equal(cp, ocp) and np.array_equal(ri, ori) and np.array_equal(cv, ocv)

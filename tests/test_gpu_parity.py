@@ -1,0 +1,294 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
+
+Bars: analysis arrays (in-degree, levels, level sets, CSC) bit-exact; FP64 x bitwise equal to
+the serial oracle for the exact-order kernels (row/level pull, level-set), relative error
+<= 1e-12 for the atomic-order push kernel and the entry-split kernel; FP32 x within
+2e-5 relative error on well-conditioned factors."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from oracle import oracle as O
+from sptrsv_amd import capi, gen
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+EXACT = [capi.VAR_ROW_PULL, capi.VAR_LEVEL_PULL, capi.VAR_LEVELSET]
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def small_matrices():
+    rp5 = np.array([0, 2, 3, 4, 5, 6]); c5 = np.array([0, 3, 1, 2, 3, 4]); v5 = np.array([1, 2, 3, 4, 5, 6.0])
+    mats = {"csr_5": gen.UpperCSR(5, rp5, c5.astype(np.int32), v5)}
+    mats["zipf_3000"] = gen.random_zipf(3000, seed=11)
+    mats["ic0_12"] = gen.laplace3d_ic0(12)
+    mats["chol2d_15"] = gen.laplace2d_cholesky(15)
+    mats["snode_10"] = gen.random_supernodal(10, seed=3)
+    mats["diag_only"] = gen.UpperCSR(257, np.arange(258), np.arange(257, dtype=np.int32),
+                                     np.linspace(1, 2, 257))
+    n = 70  # dense upper triangle
+    rows, cols = np.triu_indices(n)
+    rng = np.random.default_rng(5)
+    vals = rng.uniform(-0.05, 0.05, len(rows))
+    vals[rows == cols] = rng.uniform(1, 2, n)
+    mats["dense_70"] = gen._from_rows(n, rows, cols, vals, "dense_70")
+    n = 4000  # bidiagonal chain
+    rows = np.concatenate([np.arange(n), np.arange(n - 1)])
+    cols = np.concatenate([np.arange(n), np.arange(1, n)])
+    vals = np.concatenate([np.full(n, 2.0), np.full(n - 1, -0.9)])
+    o = np.lexsort((cols, rows))
+    mats["chain_4000"] = gen._from_rows(n, rows[o], cols[o], vals[o], "chain")
+    return mats
+
+
+MATS = small_matrices()
+
+
+@pytest.fixture(scope="module")
+def plans():
+    out = {}
+    for k, U in MATS.items():
+        banner(f"plan {k} n={U.n} nnz={U.nnz}")
+        out[k] = capi.Plan.from_upper(U)
+    yield out
+    for p in out.values():
+        p.close()
+
+
+def oracle_solve(U, b_rowmajor):
+    """oracle serial solve; b row-major (n,) or (n, m) -> x same layout"""
+    if b_rowmajor.ndim == 1:
+        return O.base_gauss_upper(U.rowptr, U.col, U.val, b_rowmajor)[0]
+    x, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, np.ascontiguousarray(b_rowmajor.T))
+    return np.ascontiguousarray(x.T)
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_analysis_bit_exact(plans, name):
+    U, P = MATS[name], plans[name]
+    banner(f"analysis {name}")
+    lv, L = O.levels(U.rowptr, U.col)
+    assert P.n_levels == L
+    assert np.array_equal(P.levels(), lv.astype(np.int32))
+    ptr, rows = O.level_sets(lv, L)
+    gptr, grows = P.level_sets()
+    assert np.array_equal(gptr, ptr) and np.array_equal(grows, rows)
+    cp, ri, cv = O.transpose(U.rowptr, U.col, U.val)
+    gcp, gri, gcv = P.csc()
+    assert np.array_equal(gcp, cp) and np.array_equal(gri, ri) and np.array_equal(gcv, cv)
+    assert np.array_equal(P.indegree(), O.indegree_csc(U.n, ri))
+
+
+def test_device_transpose_general_pattern():
+    rng = np.random.default_rng(1)
+    import scipy.sparse as sp
+    A = sp.random(300, 300, density=0.03, random_state=rng, format="csr")
+    A.sort_indices()
+    cp, ri, cv = capi.csr_to_csc(300, A.indptr, A.indices, A.data)
+    C = A.tocsc()
+    C.sort_indices()
+    assert np.array_equal(cp, C.indptr) and np.array_equal(ri, C.indices) and np.array_equal(cv, C.data)
+
+
+@pytest.mark.parametrize("variant", EXACT + [capi.VAR_CSC_PUSH])
+@pytest.mark.parametrize("name", list(MATS))
+def test_fp64_single_rhs(plans, name, variant):
+    U, P = MATS[name], plans[name]
+    banner(f"solve m=1 {name} {capi.VARIANT_NAMES[variant]}")
+    xt, b = gen.rhs_for(U, 1, seed=2)
+    x = P.solve_host(b, capi.params(variant=variant))
+    ref = oracle_solve(U, b)
+    if variant in EXACT:
+        assert np.array_equal(x, ref), O.rel_error(ref, x)
+    else:
+        assert O.rel_error(ref, x) <= FP64_TOL
+
+
+@pytest.mark.parametrize("group", [2, 8, 64])
+@pytest.mark.parametrize("name", ["zipf_3000", "chol2d_15", "dense_70", "chain_4000"])
+def test_fp64_entry_split(plans, name, group):
+    U, P = MATS[name], plans[name]
+    banner(f"solve split {name} G={group}")
+    xt, b = gen.rhs_for(U, 1, seed=3)
+    for variant in (capi.VAR_ROW_PULL, capi.VAR_LEVEL_PULL):
+        x = P.solve_host(b, capi.params(variant=variant, group=group))
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 4])
+def test_fp64_unroll(plans, unroll):
+    for name in ("zipf_3000", "chol2d_15"):
+        U, P = MATS[name], plans[name]
+        xt, b = gen.rhs_for(U, 1, seed=4)
+        x = P.solve_host(b, capi.params(variant=capi.VAR_ROW_PULL, unroll=unroll))
+        assert np.array_equal(x, oracle_solve(U, b))
+
+
+@pytest.mark.parametrize("m", [2, 3, 8, 32, 64, 100, 128, 300])
+@pytest.mark.parametrize("variant", EXACT + [capi.VAR_CSC_PUSH])
+def test_fp64_multi_rhs(plans, m, variant):
+    for name in ("zipf_3000", "ic0_12", "snode_10"):
+        U, P = MATS[name], plans[name]
+        banner(f"solve m={m} {name} {capi.VARIANT_NAMES[variant]}")
+        xt, b = gen.rhs_for(U, m, seed=5, layout="row")
+        x = P.solve_host(b, capi.params(variant=variant))
+        ref = oracle_solve(U, b)
+        if variant in EXACT:
+            assert np.array_equal(x, ref), O.rel_error(ref, x)
+        else:
+            assert O.rel_error(ref, x) <= FP64_TOL
+
+
+def test_column_major_host_layout(plans):
+    U, P = MATS["zipf_3000"], plans["zipf_3000"]
+    xt, b = gen.rhs_for(U, 4, seed=6, layout="col")  # (m, n)
+    x = P.solve_host(b, capi.params(variant=capi.VAR_LEVEL_PULL), col_major=True)
+    ref, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
+    assert np.array_equal(x, ref)
+
+
+@pytest.mark.parametrize("variant", EXACT + [capi.VAR_CSC_PUSH])
+def test_fp32(plans, variant):
+    for name in ("zipf_3000", "ic0_12", "chol2d_15"):
+        U, P = MATS[name], plans[name]
+        for m in (1, 16):
+            xt, b = gen.rhs_for(U, m, seed=7, layout="row")
+            x = P.solve_host(b, capi.params(variant=variant, precision=capi.FP32))
+            assert O.rel_error(oracle_solve(U, b), x) <= FP32_TOL
+
+
+def test_device_pointers_and_residual(plans):
+    U, P = MATS["zipf_3000"], plans["zipf_3000"]
+    xt, b = gen.rhs_for(U, 5, seed=8, layout="row")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    d_r = capi.DeviceArray(b.shape)
+    for _ in range(3):  # repeated solves on the same buffers reuse the plan state
+        P.solve(d_b, d_x, 5, capi.params(variant=capi.VAR_LEVEL_PULL))
+    P.sync()
+    x = d_x.download()
+    assert np.array_equal(x, oracle_solve(U, b))
+    P.residual(d_b, d_x, d_r, 5)
+    P.sync()
+    r = d_r.download()
+    rref = O.spmv_residual(U.rowptr, U.col, U.val, np.ascontiguousarray(x.T), np.ascontiguousarray(b.T)).T
+    np.testing.assert_allclose(r, rref, rtol=0, atol=1e-13)
+
+
+def test_watchdog_reports_and_recovers(plans):
+    """A 1 us per-wait limit on a 4000-long chain forces the kernel's own bounded wait to give
+    up (every lane exits normally); the host sees ERR_DEADLOCK, and the next solve is fine."""
+    U, P = MATS["chain_4000"], plans["chain_4000"]
+    xt, b = gen.rhs_for(U, 1, seed=9)
+    with pytest.raises(capi.SptrsvError) as e:
+        P.solve_host(b, capi.params(variant=capi.VAR_ROW_PULL, watchdog_us=1))
+    assert e.value.code == capi.ERR_DEADLOCK
+    x = P.solve_host(b, capi.params(variant=capi.VAR_ROW_PULL))
+    assert np.array_equal(x, oracle_solve(U, b))
+
+
+def test_full_size_ic0_128():
+    """configs: 7-point IC(0) on 128^3 (n = 2,097,152): closed-form analysis and FP64 parity."""
+    U = gen.laplace3d_ic0(128)
+    banner("plan ic0_128")
+    with capi.Plan.from_upper(U) as P:
+        assert P.n_levels == 382
+        N = 128
+        idx = np.arange(U.n)
+        x, y, z = idx % N, (idx // N) % N, idx // (N * N)
+        assert np.array_equal(P.levels(), (1 + (N - 1 - x) + (N - 1 - y) + (N - 1 - z)).astype(np.int32))
+        assert np.array_equal(P.indegree(), (1 + (x < N - 1) + (y < N - 1) + (z < N - 1)).astype(np.int32))
+        xt, b = gen.rhs_for(U, 1, seed=1)
+        ref = oracle_solve(U, b)
+        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL):
+            banner(f"solve ic0_128 {capi.VARIANT_NAMES[variant]}")
+            got = P.solve_host(b, capi.params(variant=variant))
+            assert np.array_equal(got, ref)
+        got = P.solve_host(b, capi.params(variant=capi.VAR_CSC_PUSH))
+        assert O.rel_error(ref, got) <= FP64_TOL
+        assert O.rel_error(xt, ref) < 1e-10
+
+
+def test_full_size_zipf_1e6_level_pull():
+    U = gen.random_zipf(1_000_000, seed=1)
+    banner("plan zipf_1e6")
+    with capi.Plan.from_upper(U) as P:
+        lv, L = O.levels(U.rowptr, U.col)
+        assert P.n_levels == L
+        assert np.array_equal(P.levels(), lv.astype(np.int32))
+        xt, b = gen.rhs_for(U, 1, seed=1)
+        got = P.solve_host(b, capi.params(variant=capi.VAR_ROW_PULL))
+        assert np.array_equal(got, oracle_solve(U, b))
+
+
+def _build(src, out):
+    lib_dir = os.path.dirname(capi.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", src), "-o", out, "-L", lib_dir, "-lsptrsv_amd",
+                    f"-Wl,-rpath,{lib_dir}"], check=True)
+    return out
+
+
+def _write_txt(path, U):
+    with open(path, "w") as f:
+        f.write(f"{U.n} {U.nnz}\n")
+        f.write(" ".join(repr(float(v)) for v in U.val) + "\n")
+        f.write(" ".join(str(int(c)) for c in U.col) + "\n")
+        f.write(" ".join(str(int(r)) for r in U.rowptr) + "\n")
+
+
+@pytest.mark.parametrize("rhs", [1, 3])
+@pytest.mark.parametrize("which", ["csr_6", "snode_6"])
+def test_cpp_dropin_entry_points(tmp_path, which, rhs):
+    """Every reference entry point (include/core.h, include/sptrsv_syncfree_opencl.h) called
+    from C++ exactly as the reference's run() calls it, checked against the oracle."""
+    exe = _build("dropin_solve.cpp", str(tmp_path / "dropin_solve"))
+    if which == "csr_6":
+        mpath = os.path.join(GOLDEN, "csr_6_upper.txt")
+        spath = os.path.join(GOLDEN, "csr_6_upper_snodes.txt")
+        tok = open(mpath).read().split()
+        n, nz = int(tok[0]), int(tok[1])
+        U = gen.UpperCSR(n, np.array([int(t) for t in tok[2 + 2 * nz:]]),
+                         np.array([int(t) for t in tok[2 + nz:2 + 2 * nz]], dtype=np.int32),
+                         np.array([float(t) for t in tok[2:2 + nz]]))
+        snodes = np.array([0, 1, 3, 4, 6], dtype=np.int32)
+    else:
+        U = gen.random_supernodal(6, seed=4, wmin=4, wmax=12)
+        mpath, spath = str(tmp_path / "m.txt"), str(tmp_path / "s.txt")
+        _write_txt(mpath, U)
+        with open(spath, "w") as f:
+            f.write(f"{len(U.snodes) - 1}\n" + " ".join(str(int(s)) for s in U.snodes) + "\n")
+        snodes = U.snodes
+    out = subprocess.run([exe, mpath, spath, str(rhs), str(tmp_path)], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = U.n
+    cnt = np.arange(n * rhs)
+    b = (1.0 + (cnt * 7) % 11).reshape(rhs, n)  # column-major as written by the driver
+    ref, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
+    load = lambda k: np.fromfile(str(tmp_path / f"{k}.bin"), dtype=np.float64)
+    for k in ("base", "base_new", "barrier"):
+        assert np.array_equal(load(k), ref.ravel()), k
+    for k in ("custom", "blas"):
+        xs, _ = O.supernodal_upper(U.rowptr, U.col, U.val, snodes, b)
+        assert O.rel_error(xs.ravel(), load(k)) <= FP64_TOL, k
+    lv, L = O.levels(U.rowptr, U.col)
+    assert np.array_equal(load("levels"), lv.astype(np.float64))
+    assert f"levels {L}" in out.stdout
+    ref_row = np.ascontiguousarray(ref.T).ravel()
+    assert O.rel_error(ref_row, load("syncfree")) <= FP64_TOL
+    if rhs == 1:
+        assert O.rel_error(ref_row, load("syncfree2")) <= FP64_TOL
+        assert np.array_equal(load("write_first"), ref_row)
+        cp, ri, cv = O.transpose(U.rowptr, U.col, U.val)
+        lo, _ = O.base_gauss_lower(cp.astype(np.uint64), ri, cv, b[0])
+        assert O.rel_error(lo, load("lower")) <= FP64_TOL
+    assert "nonsquare -1.0" in out.stdout

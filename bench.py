@@ -45,9 +45,13 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def time_solves(capi, P, d_b, d_x, m, prm, steps, warmup):
-    for _ in range(warmup):
+def time_solves(capi, P, d_b, d_x, m, prm, steps, warmup, first=None):
+    for i in range(warmup):
+        t0 = time.perf_counter()
         P.solve(d_b, d_x, m, prm)
+        if first is not None:
+            P.sync()
+            first.append(round((time.perf_counter() - t0) * 1e3, 3))
     P.sync()
     capi.device_sync()
     P.timing(True)
@@ -128,7 +132,8 @@ def main():
     d_b = capi.DeviceArray.from_host(b)
     d_x = capi.DeviceArray(b.shape)
     prm = capi.params(variant=variant)
-    ms, kms = time_solves(capi, P, d_b, d_x, m, prm, args.steps, args.warmup)
+    first = []
+    ms, kms = time_solves(capi, P, d_b, d_x, m, prm, args.steps, args.warmup, first)
     x = d_x.download()
     bb = b if m == 1 else np.ascontiguousarray(b.T)
     ref, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, bb)
@@ -140,8 +145,9 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        runs = [("ic0_128", "csc_push", 1), ("ic0_128", "row_pull", 1), ("ic0_128", "level_pull", 32),
-                ("ic0_128", "level_pull", 128), ("zipf_1e6", "row_pull", 1), ("zipf_1e6", "level_pull", 1)]
+        runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
+                ("ic0_128", "level_pull", 32), ("ic0_128", "level_pull", 128), ("zipf_1e6", "tile", 1),
+                ("zipf_1e6", "row_pull", 1), ("chol2d_100", "tile", 1), ("chol2d_100", "row_pull", 1)]
         cache = {args.workload: (U, P)}
         for wl, vname, mm in runs:
             if (wl, vname, mm) == (args.workload, args.variant, m):
@@ -154,12 +160,13 @@ def main():
                 _, bx = gen.rhs_for(UU, mm, seed=1, layout="row")
                 db = capi.DeviceArray.from_host(bx)
                 dx = capi.DeviceArray(bx.shape)
+                e_first = []
                 e_ms, e_kms = time_solves(capi, PP, db, dx, mm, capi.params(variant=names[vname]),
-                                          3 if mm > 1 or wl != "ic0_128" else 10, 1)
+                                          3 if mm > 1 or wl != "ic0_128" else 10, 2, e_first)
                 BB = alg_bytes(UU.n, UU.nnz, mm)
                 extras[f"{wl}/{vname}/m{mm}"] = {"ms_per_step": round(e_ms, 4), "kernel_ms": round(e_kms, 4),
                                                  "GB_s": round(BB / (e_ms / 1e3) / 1e9, 2),
-                                                 "nnz_per_s": UU.nnz * mm / (e_ms / 1e3)}
+                                                 "nnz_per_s": UU.nnz * mm / (e_ms / 1e3), "first_ms": e_first}
                 db.free()
                 dx.free()
             except Exception as exc:  # report, never hide
@@ -184,6 +191,7 @@ def main():
         "pct_hbm_peak": round(100.0 * gbs / HBM_PEAK_GBS, 3),
         "nnz_per_s": U.nnz * m / (ms / 1e3),
         "rel_err_vs_oracle": err,
+        "warmup_solves_ms": first,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args.workload, args.variant, m),
                      "kernel_ms": round(kms, 5), "alg_bytes": B},

@@ -46,7 +46,14 @@ extern "C" {
 #define SPTRSV_VAR_LEVELSET 4     /* bulk-synchronous: one launch per level (src/core.cpp:273-376) */
 #define SPTRSV_VAR_WINDOW 5       /* wave per 64 consecutive rows, in-window dependencies resolved
                                      through cross-lane broadcasts (chains) */
-#define SPTRSV_VAR_COUNT 6
+#define SPTRSV_VAR_SUPERNODAL 6   /* one workgroup per supernode: dense blocks on MFMA, the
+                                     diagonal block applied through its precomputed inverse
+                                     (needs sptrsv_plan_build_supernodes) */
+#define SPTRSV_VAR_TILE 7         /* one 1024-thread workgroup per tile of <= 960 consecutive rows:
+                                     entries staged in LDS, dependencies inside the tile resolved
+                                     through LDS, outside ones fetched by a poller wave; m == 1,
+                                     exact reference summation order */
+#define SPTRSV_VAR_COUNT 8
 
 /* Value precision of the solve. */
 #define SPTRSV_FP64 0
@@ -99,6 +106,15 @@ int sptrsv_plan_create_csc(int n, int64_t nnz, const int* colptr, const int* row
                            const double* val, sptrsv_plan_t* plan);
 int sptrsv_plan_destroy(sptrsv_plan_t plan);
 int sptrsv_plan_info(sptrsv_plan_t plan, int* n, int64_t* nnz, int* n_levels);
+
+/* Supernodal layout.  snodes[0..sn] are row boundaries (reference .snodes semantics,
+ * src/utils.cpp:84-111); every supernode's rows must hold a dense triangle followed by one
+ * common set of columns (the structure src/core.cpp:53-78 assumes).  sn <= 0: detect the
+ * fundamental supernodes from the pattern.  Supernodes wider than 128 rows are split (the
+ * split keeps the structure).  Diagonal blocks are inverted on the device. */
+int sptrsv_plan_build_supernodes(sptrsv_plan_t plan, int sn, const int* snodes);
+/* Number of supernodes in use and (optionally, sn+1 entries) their boundaries. */
+int sptrsv_plan_get_supernodes(sptrsv_plan_t plan, int* sn, int* snodes);
 
 /* Analysis results (device-computed, copied to host). */
 int sptrsv_plan_get_indegree(sptrsv_plan_t plan, int* indeg /* n */);

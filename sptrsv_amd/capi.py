@@ -28,8 +28,11 @@ VAR_ROW_PULL = 2
 VAR_LEVEL_PULL = 3
 VAR_LEVELSET = 4
 VAR_WINDOW = 5
+VAR_SUPERNODAL = 6
+VAR_TILE = 7
 VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
-                 VAR_LEVELSET: "levelset", VAR_WINDOW: "window"}
+                 VAR_LEVELSET: "levelset", VAR_WINDOW: "window",
+                 VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile"}
 
 FP64 = 0
 FP32 = 1
@@ -93,6 +96,8 @@ def lib():
         "sptrsv_plan_timing": (_i, [_PLAN, _i]),
         "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
         "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
+        "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
+        "sptrsv_plan_get_supernodes": (_i, [_PLAN, ctypes.POINTER(_i), _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,7 +218,9 @@ class Plan:
         c = np.ascontiguousarray(col, dtype=np.int32)
         v = np.ascontiguousarray(val, dtype=np.float64)
         h = _PLAN()
-        check(lib().sptrsv_plan_create_csr(len(rp) - 1, int(rp[-1]), _ptr(rp), _ptr(c), _ptr(v),
+        if len(c) != int(rp[-1]) or len(v) != len(c):
+            raise SptrsvError(ERR_BAD_INPUT, "plan_create_csr", "array lengths do not match rowptr")
+        check(lib().sptrsv_plan_create_csr(len(rp) - 1, len(c), _ptr(rp), _ptr(c), _ptr(v),
                                            ctypes.byref(h)), "plan_create_csr")
         return cls(h)
 
@@ -264,6 +271,22 @@ class Plan:
         rows = np.zeros(self.n, dtype=np.int32)
         check(lib().sptrsv_plan_get_level_sets(self.handle, _ptr(ptr), _ptr(rows)), "get_level_sets")
         return ptr, rows
+
+    def build_supernodes(self, snodes=None):
+        """Supernodal layout from boundaries `snodes` (sn+1 ints), or detected when None."""
+        if snodes is None:
+            check(lib().sptrsv_plan_build_supernodes(self.handle, 0, None), "build_supernodes")
+        else:
+            s = np.ascontiguousarray(snodes, dtype=np.int32)
+            check(lib().sptrsv_plan_build_supernodes(self.handle, len(s) - 1, _ptr(s)), "build_supernodes")
+        return self.supernodes()
+
+    def supernodes(self):
+        sn = _i(0)
+        check(lib().sptrsv_plan_get_supernodes(self.handle, ctypes.byref(sn), None), "get_supernodes")
+        out = np.zeros(sn.value + 1, dtype=np.int32)
+        check(lib().sptrsv_plan_get_supernodes(self.handle, ctypes.byref(sn), _ptr(out)), "get_supernodes")
+        return out
 
     def csc(self):
         cp = np.zeros(self.n + 1, dtype=np.int32)

@@ -77,6 +77,20 @@ __device__ __forceinline__ int wave_ticket(int* counter) {
     return __shfl(t, 0, kWave);
 }
 
+// s_waitcnt vmcnt(0): on gfx950 one in-order counter covers loads AND stores, so a load whose
+// first use sits inside a polling loop would make every later use wait for the wave's
+// outstanding x stores.  Loads needed inside such loops are completed before the loop.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Workgroup-level ticket: one atomic per workgroup instead of one per wave (a single shared
+// counter serialises its atomics, so fewer, larger tickets matter at millions of rows).
+__device__ __forceinline__ int block_ticket(int* counter) {
+    __shared__ int s_ticket;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1);
+    __syncthreads();
+    return s_ticket;
+}
+
 // Watchdog: a lane that has waited longer than `limit` wall-clock ticks sets the status
 // word and gives up; every waiting lane also gives up once the status word is non-zero,
 // so a malformed input ends the kernel instead of hanging the device.

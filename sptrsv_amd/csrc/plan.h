@@ -48,6 +48,39 @@ struct sptrsv_plan_s {
 
     std::vector<int> h_level_ptr;  // host copy, drives the bulk-synchronous launches
 
+    // supernodal layout (built by sptrsv_plan_build_supernodes): supernode k holds rows
+    // [sn_start[k], sn_start[k+1]); its diagonal block is stored INVERTED as a dense row-major
+    // w x w block, its rectangular part as a dense row-major w x r block over the columns
+    // sn_cols[sn_offc[k] .. + r).
+    int sn = 0;
+    int sn_wmax = 0;
+    int* sn_start = nullptr;     // sn + 1
+    int* sn_r = nullptr;         // sn
+    int64_t* sn_offt = nullptr;  // sn, offset of the inverse block
+    int64_t* sn_offr = nullptr;  // sn, offset of the rectangular block
+    int* sn_offc = nullptr;      // sn, offset into sn_cols
+    int* sn_cols = nullptr;
+    double* sn_tinv = nullptr;
+    double* sn_rect = nullptr;
+    float* sn_tinv32 = nullptr;     // mixed precision: FP32 inverse blocks
+    uint16_t* sn_rect16 = nullptr;  // mixed precision: BF16 rectangular blocks, scaled per block
+    float* sn_rscale = nullptr;     // per-supernode scale of the BF16 block
+    int* sn_dep_ptr = nullptr;   // sn + 1
+    int* sn_dep = nullptr;       // supernodes owning the rectangular columns
+    int* sn_flag = nullptr;      // sn, epoch of the last solve that published the supernode
+    int sn_epoch = 0;
+    std::vector<int> h_snodes;
+
+    // tiles for the TILE variant (built on first use): tile t covers rows [tile_lo[t],
+    // tile_hi[t]) (tile 0 = last rows); tile_ext[tile_ext_ptr[t]..] are the columns outside the
+    // tile that its rows read (descending); lidx[p] is the LDS slot of CSR entry p.
+    int n_tiles = 0;
+    int* tile_lo = nullptr;
+    int* tile_hi = nullptr;
+    int* tile_ext_ptr = nullptr;
+    int* tile_ext = nullptr;
+    uint16_t* tile_lidx = nullptr;
+
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -60,6 +93,12 @@ int analyse(sptrsv_plan_s* p);           // analysis.hip
 int ensure_fp32(sptrsv_plan_s* p);       // analysis.hip
 int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, const double* val,
                      int* colptr, int* rowidx, double* cval, hipStream_t s);  // analysis.hip
+int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
+                     uint64_t limit);  // supernodal.hip
+void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip
+int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
+               uint64_t limit);  // tile.hip
+void tile_free(sptrsv_plan_s* p);         // tile.hip
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);
 void end_timed(sptrsv_plan_s* p, hipEvent_t stop);
 }  // namespace sptrsv

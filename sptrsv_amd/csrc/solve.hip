@@ -44,14 +44,18 @@ struct PullArgs {
     uint64_t limit;    // watchdog ticks
 };
 
-template <typename T, int G, int KPL, int UNROLL, bool LEVELED>
-__global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
+// Workgroup size of the pull kernels: 256 threads for one lane per row (finer tickets keep the
+// dependency wavefront dense), 1024 when several lanes share a row (fewer ticket atomics).
+constexpr int kPullBlock = 1024;
+constexpr int kPullBlockRow = 256;
+
+template <typename T, int G, int KPL, int UNROLL, bool LEVELED, int BS>
+__global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
 #pragma clang fp contract(off)
     using U = typename Bits<T>::U;
-    constexpr int RPW = kWave / G;
     const int lane = threadIdx.x & (kWave - 1);
-    const int t = wave_ticket(&a.ctrl[0]);
-    const int slot = t * RPW + lane / G;
+    const int t = block_ticket(&a.ctrl[0]);
+    const int slot = t * (BS / G) + (int)threadIdx.x / G;
     const int gl = lane % G;
     bool done = slot >= a.n || gl >= a.kc;
     int row_csr = 0, row_out = 0, p = 0, end = 0;
@@ -59,6 +63,10 @@ __global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
     T acc[KPL];
 #pragma unroll
     for (int j = 0; j < KPL; ++j) acc[j] = T(0);
+    const size_t m = (size_t)a.m;
+    T bv[KPL];
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) bv[j] = T(0);
     if (!done) {
         row_csr = LEVELED ? slot : a.n - 1 - slot;
         row_out = LEVELED ? a.perm[slot] : row_csr;
@@ -66,8 +74,11 @@ __global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
         diag = a.val[s];
         p = s + 1;
         end = a.rowptr[row_csr + 1];
+#pragma unroll
+        for (int j = 0; j < KPL; ++j)
+            if (gl + j * G < a.kc) bv[j] = a.b[(size_t)row_out * m + a.k0 + gl + j * G];
     }
-    const size_t m = (size_t)a.m;
+    wait_vmem();
     Watchdog wd;
     wd.arm(&a.ctrl[1], a.limit);
     int idle = 0;
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
                     const int k = gl + j * G;
                     if (k < a.kc) {
                         const size_t at = (size_t)row_out * m + a.k0 + k;
-                        publish(a.x + at, (a.b[at] - acc[j]) / diag);
+                        publish(a.x + at, (bv[j] - acc[j]) / diag);
                     }
                 }
                 done = true;
@@ -142,17 +153,16 @@ __global__ __launch_bounds__(256) void k_pull(PullArgs<T> a) {
 // (cross-lane shuffles need every lane active), and a group whose lanes have all finished
 // their share publishes once.
 template <typename T, int G, bool LEVELED>
-__global__ __launch_bounds__(256) void k_pull_split(PullArgs<T> a) {
+__global__ __launch_bounds__(kPullBlock) void k_pull_split(PullArgs<T> a) {
     using U = typename Bits<T>::U;
-    constexpr int RPW = kWave / G;
     const int lane = threadIdx.x & (kWave - 1);
-    const int t = wave_ticket(&a.ctrl[0]);
-    const int slot = t * RPW + lane / G;
+    const int t = block_ticket(&a.ctrl[0]);
+    const int slot = t * (kPullBlock / G) + (int)threadIdx.x / G;
     const int gl = lane % G;
     const bool has_row = slot < a.n;
     bool share_done = !has_row, published = !has_row;
     int row_out = 0, p = 0, end = 0;
-    T diag = T(1), acc = T(0);
+    T diag = T(1), acc = T(0), bv = T(0);
     if (has_row) {
         const int row_csr = LEVELED ? slot : a.n - 1 - slot;
         row_out = LEVELED ? a.perm[slot] : row_csr;
@@ -160,7 +170,9 @@ __global__ __launch_bounds__(256) void k_pull_split(PullArgs<T> a) {
         diag = a.val[s];
         p = s + 1 + gl;
         end = a.rowptr[row_csr + 1];
+        bv = a.b[row_out];
     }
+    wait_vmem();
     const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << ((lane / G) * G));
     Watchdog wd;
     wd.arm(&a.ctrl[1], a.limit);
@@ -182,7 +194,7 @@ __global__ __launch_bounds__(256) void k_pull_split(PullArgs<T> a) {
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, G);
         if (group_done && !published) {
-            if (gl == 0) publish(a.x + row_out, (a.b[row_out] - tot) / diag);
+            if (gl == 0) publish(a.x + row_out, (bv - tot) / diag);
             published = true;
             progressed = true;
         }
@@ -335,21 +347,20 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
     const int n = P->n;
     if (m == 1 && group > 1) {
         const int G = next_pow2(std::min(group, 64));
-        const int waves = (int)(((int64_t)n * G + kWave - 1) / kWave);
         PullArgs<T> a = base;
         a.k0 = 0;
         a.kc = 1;
         hipEvent_t stop;
         SPT_CHECK(begin_timed(P, &stop));
         k_reset_ticket<<<1, 64, 0, s>>>(a.ctrl);
-        const dim3 grid((waves + 3) / 4);
+        const dim3 grid((unsigned)(((int64_t)n * G + kPullBlock - 1) / kPullBlock));
         switch (G) {
-            case 2: k_pull_split<T, 2, LEVELED><<<grid, 256, 0, s>>>(a); break;
-            case 4: k_pull_split<T, 4, LEVELED><<<grid, 256, 0, s>>>(a); break;
-            case 8: k_pull_split<T, 8, LEVELED><<<grid, 256, 0, s>>>(a); break;
-            case 16: k_pull_split<T, 16, LEVELED><<<grid, 256, 0, s>>>(a); break;
-            case 32: k_pull_split<T, 32, LEVELED><<<grid, 256, 0, s>>>(a); break;
-            default: k_pull_split<T, 64, LEVELED><<<grid, 256, 0, s>>>(a); break;
+            case 2: k_pull_split<T, 2, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
+            case 4: k_pull_split<T, 4, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
+            case 8: k_pull_split<T, 8, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
+            case 16: k_pull_split<T, 16, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
+            case 32: k_pull_split<T, 32, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
+            default: k_pull_split<T, 64, LEVELED><<<grid, kPullBlock, 0, s>>>(a); break;
         }
         end_timed(P, stop);
         SPT_HIP(hipGetLastError());
@@ -362,17 +373,18 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
         a.kc = kc;
         const int G = kc == 1 ? 1 : std::min(64, next_pow2(kc));
         const int kpl = (kc + G - 1) / G;  // 1..4
-        const int rows_per_wave = kWave / G;
-        const int waves = (n + rows_per_wave - 1) / rows_per_wave;
-        const dim3 grid((waves + 3) / 4);
+        const int bs = G == 1 ? kPullBlockRow : kPullBlock;
+        const int rows_per_block = bs / G;
+        const dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block));
         hipEvent_t stop;
         SPT_CHECK(begin_timed(P, &stop));
         k_reset_ticket<<<1, 64, 0, s>>>(a.ctrl);
-#define SPT_PULL(G_, K_, U_) k_pull<T, G_, K_, U_, LEVELED><<<grid, 256, 0, s>>>(a)
+#define SPT_PULL(G_, K_, U_) k_pull<T, G_, K_, U_, LEVELED, kPullBlock><<<grid, kPullBlock, 0, s>>>(a)
+#define SPT_PULL1(U_) k_pull<T, 1, 1, U_, LEVELED, kPullBlockRow><<<grid, kPullBlockRow, 0, s>>>(a)
         if (G == 1) {
-            if (unroll == 1) SPT_PULL(1, 1, 1);
-            else if (unroll == 2) SPT_PULL(1, 1, 2);
-            else SPT_PULL(1, 1, 4);
+            if (unroll == 1) SPT_PULL1(1);
+            else if (unroll == 2) SPT_PULL1(2);
+            else SPT_PULL1(4);
         } else if (G == 2) SPT_PULL(2, 1, 2);
         else if (G == 4) SPT_PULL(4, 1, 2);
         else if (G == 8) SPT_PULL(8, 1, 2);
@@ -382,6 +394,7 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
         else if (kpl == 2) SPT_PULL(64, 2, 1);
         else SPT_PULL(64, 4, 1);
 #undef SPT_PULL
+#undef SPT_PULL1
         end_timed(P, stop);
         SPT_HIP(hipGetLastError());
     }
@@ -490,6 +503,8 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
                                             : P->ticks_per_ms * 20000ull;
     const int unroll = prm.unroll > 0 ? prm.unroll : 4;
+    if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
+    if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, d_b, d_x, m, limit);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, d_b, d_x, m, P->val, P->cval, P->lval, limit);
     if (prm.precision == SPTRSV_FP32) {

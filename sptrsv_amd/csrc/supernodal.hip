@@ -1,0 +1,339 @@
+// supernodal.hip -- supernodal U X = B on FP64 MFMA (gfx950).
+//
+// Reference semantics: src/core.cpp:53-201 (supernodal_upper, ccs2ccs_pad,
+// supernodal_blas_upper): rows of a supernode share a dense triangle plus one common set of
+// "rectangular" columns; the solve goes supernode by supernode from the last one, first
+// x_s -= R_s x(rect) (the reference's cblas_dgemm), then the triangular solve with the
+// diagonal block (its cblas_dtrsm).
+//
+// gfx950 design: the diagonal block of every supernode is inverted once, on the device, when
+// the layout is built, so a supernode's solve is two dense products on MFMA:
+//     W   = B_s - R_s * X(rect, :)       (w x m, K = r)      v_mfma_f64_16x16x4f64
+//     X_s = Tinv_s * W                   (w x m, K = w)      v_mfma_f64_16x16x4f64
+// v_mfma_f64_16x16x4f64 lane layout (measured on gfx950): A(i,k) in lane i + 16k, B(k,j) in
+// lane j + 16k, result element v of lane l is D(l/16 + 4v, l%16).
+// One 256-thread workgroup per supernode, handed out by tickets from the last supernode
+// down; a workgroup waits (acquire) for the epoch flags of the supernodes owning its
+// rectangular columns -- all later in ticket order, so already running -- and publishes its
+// own flag (release) after writing X_s.  W lives in LDS (w <= 128 rows x MC columns).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace sptrsv {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxW = 128;  // widest supernode the kernel holds; wider ones are split
+
+// column j of Tinv by back-substitution on T e_j (one thread per column)
+__global__ void k_sn_invert(const int* __restrict__ start, const int64_t* __restrict__ offt,
+                            const double* __restrict__ T, double* __restrict__ Ti) {
+    const int k = blockIdx.x;
+    const int s0 = start[k], w = start[k + 1] - s0;
+    const int64_t base = offt[k];
+    for (int j = threadIdx.x; j < w; j += blockDim.x) {
+        for (int i = j + 1; i < w; ++i) Ti[base + (int64_t)i * w + j] = 0.0;
+        Ti[base + (int64_t)j * w + j] = 1.0 / T[base + (int64_t)j * w + j];
+        for (int i = j - 1; i >= 0; --i) {
+            double acc = 0.0;
+            for (int e = i + 1; e <= j; ++e) acc = fma(T[base + (int64_t)i * w + e], Ti[base + (int64_t)e * w + j], acc);
+            Ti[base + (int64_t)i * w + j] = -acc / T[base + (int64_t)i * w + i];
+        }
+    }
+}
+
+struct SnArgs {
+    int sn;
+    int m;
+    const int* start;
+    const int* rcount;
+    const int64_t* offt;
+    const int64_t* offr;
+    const int* offc;
+    const int* cols;
+    const double* tinv;
+    const double* rect;
+    const int* dep_ptr;
+    const int* dep;
+    int* flag;
+    int epoch;
+    const double* b;
+    double* x;
+    int* ctrl;
+    uint64_t limit;
+};
+
+template <int MC>
+__global__ __launch_bounds__(256) void k_sn_solve(SnArgs a) {
+    __shared__ double W[kMaxW * MC];
+    __shared__ int s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
+    __syncthreads();
+    const int k = a.sn - 1 - s_ticket;
+    if (k < 0) return;
+    const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
+    if (tid == 0) {
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        int idle = 0;
+        bool give_up = false;
+        for (int q = a.dep_ptr[k]; q < a.dep_ptr[k + 1] && !give_up; ++q) {
+            const int d = a.dep[q];
+            while (__hip_atomic_load(&a.flag[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if ((++idle & 63) == 0 && wd.expired()) {
+                    give_up = true;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    const int nI = (w + 15) >> 4;
+    constexpr int nJ = MC / 16;
+    const int tiles = nI * nJ;
+    const int c16 = lane & 15, quad = lane >> 4;
+    const double* R = a.rect + a.offr[k];
+    const double* Ti = a.tinv + a.offt[k];
+    const int* C = a.cols + a.offc[k];
+    const size_t m = (size_t)a.m;
+    for (int j0 = 0; j0 < a.m; j0 += MC) {
+        // W = B_s - R_s * X(rect, j0:j0+MC)
+        for (int t = wave; t < tiles; t += 4) {
+            const int I = t / nJ, J = t % nJ;
+            const int jj = j0 + 16 * J + c16;
+            const bool jv = jj < a.m;
+            v4d acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * I + quad + 4 * v;
+                acc[v] = (i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
+            }
+            const int ia = 16 * I + c16;
+            const bool iav = ia < w;
+            const double* Rrow = R + (size_t)(iav ? ia : 0) * r;
+#pragma unroll 4
+            for (int k0 = 0; k0 < r; k0 += 4) {
+                const int kk = k0 + quad;
+                const bool kv = kk < r;
+                const double av = (iav && kv) ? -Rrow[kk] : 0.0;
+                const double bv = (kv && jv) ? a.x[(size_t)C[kv ? kk : 0] * m + jj] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) W[(16 * I + quad + 4 * v) * MC + 16 * J + c16] = acc[v];
+        }
+        __syncthreads();
+        // X_s = Tinv_s * W   (Tinv is upper triangular: row tile I only needs k >= 16 I)
+        for (int t = wave; t < tiles; t += 4) {
+            const int I = t / nJ, J = t % nJ;
+            const int jj = j0 + 16 * J + c16;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            const int ia = 16 * I + c16;
+            const bool iav = ia < w;
+            const double* Trow = Ti + (size_t)(iav ? ia : 0) * w;
+#pragma unroll 4
+            for (int k0 = 16 * I; k0 < w; k0 += 4) {
+                const int kk = k0 + quad;
+                const double av = (iav && kk < w) ? Trow[kk] : 0.0;
+                const double bv = W[kk * MC + 16 * J + c16];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * I + quad + 4 * v;
+                if (i < w && jj < a.m) a.x[(size_t)(s0 + i) * m + jj] = acc[v];
+            }
+        }
+        __syncthreads();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_reset_ticket_sn(int* ctrl) {
+    if (threadIdx.x == 0) ctrl[0] = 0;
+}
+
+template <typename X>
+int upload(X** dst, const std::vector<X>& src) {
+    SPT_HIP(hipMalloc((void**)dst, std::max<size_t>(src.size(), 1) * sizeof(X)));
+    if (!src.empty()) SPT_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(X), hipMemcpyHostToDevice));
+    return SPTRSV_OK;
+}
+
+}  // namespace
+
+void supernodal_free(sptrsv_plan_s* p) {
+    void* bufs[] = {p->sn_start, p->sn_r, p->sn_offt, p->sn_offr, p->sn_offc, p->sn_cols, p->sn_tinv,
+                    p->sn_rect, p->sn_tinv32, p->sn_rect16, p->sn_rscale, p->sn_dep_ptr, p->sn_dep, p->sn_flag};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    p->sn_start = p->sn_r = p->sn_offc = p->sn_cols = p->sn_dep_ptr = p->sn_dep = p->sn_flag = nullptr;
+    p->sn_offt = p->sn_offr = nullptr;
+    p->sn_tinv = p->sn_rect = nullptr;
+    p->sn_tinv32 = nullptr;
+    p->sn_rect16 = nullptr;
+    p->sn_rscale = nullptr;
+    p->sn = 0;
+    p->h_snodes.clear();
+}
+
+int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double* d_x, int m, uint64_t limit) {
+    if (P->sn == 0)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
+    if (precision != SPTRSV_FP64)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant: FP64 only through sptrsv_solve");
+    P->sn_epoch++;
+    SnArgs a{P->sn, m, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
+             P->sn_rect, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch, d_b, d_x, P->ticket, limit};
+    k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
+    hipEvent_t stop;
+    SPT_CHECK(begin_timed(P, &stop));
+    if (m <= 16) k_sn_solve<16><<<P->sn, 256, 0, P->stream>>>(a);
+    else if (m <= 32) k_sn_solve<32><<<P->sn, 256, 0, P->stream>>>(a);
+    else k_sn_solve<64><<<P->sn, 256, 0, P->stream>>>(a);
+    end_timed(P, stop);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const int* snodes_in) {
+    if (!P) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
+    const int n = P->n;
+    const int64_t nnz = P->nnz;
+    std::vector<int> rp((size_t)n + 1), col((size_t)nnz);
+    std::vector<double> val((size_t)nnz);
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    SPT_HIP(hipMemcpy(rp.data(), P->rowptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+
+    // ---- boundaries ----------------------------------------------------------------------
+    std::vector<int> bounds;
+    if (sn_in > 0) {
+        if (!snodes_in) return fail(SPTRSV_ERR_BAD_INPUT, "null supernode array");
+        if (snodes_in[0] != 0 || snodes_in[sn_in] != n)
+            return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must start at 0 and end at n");
+        for (int k = 0; k < sn_in; ++k)
+            if (snodes_in[k + 1] <= snodes_in[k])
+                return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must increase");
+        bounds.assign(snodes_in, snodes_in + sn_in + 1);
+    } else {
+        // fundamental supernodes: row i+1 continues row i's supernode when
+        // pattern(i) == {i} U pattern(i+1)
+        bounds.push_back(0);
+        for (int i = 0; i + 1 < n; ++i) {
+            const int li = rp[i + 1] - rp[i], lj = rp[i + 2] - rp[i + 1];
+            bool cont = li == lj + 1 && li >= 2 && col[rp[i] + 1] == i + 1;
+            for (int t = 0; cont && t < lj; ++t) cont = col[rp[i] + 1 + t] == col[rp[i + 1] + t];
+            if (!cont) bounds.push_back(i + 1);
+        }
+        bounds.push_back(n);
+    }
+    // split wide supernodes (the first part's extra columns join its rectangular set)
+    std::vector<int> sb;
+    for (size_t k = 0; k + 1 < bounds.size(); ++k)
+        for (int q = bounds[k]; q < bounds[k + 1]; q += kMaxW) sb.push_back(q);
+    sb.push_back(n);
+    const int S = (int)sb.size() - 1;
+
+    std::vector<int> sn_of((size_t)n);
+    for (int k = 0; k < S; ++k)
+        for (int i = sb[k]; i < sb[k + 1]; ++i) sn_of[(size_t)i] = k;
+    std::vector<int> rcount(S), offc(S), cols, dep_ptr(S + 1, 0), dep;
+    std::vector<int64_t> offt(S), offr(S);
+    int64_t tot_t = 0, tot_r = 0;
+    int wmax = 0;
+    for (int k = 0; k < S; ++k) {
+        const int s0 = sb[k], w = sb[k + 1] - s0;
+        wmax = std::max(wmax, w);
+        const int len0 = rp[s0 + 1] - rp[s0];
+        bool ok = len0 >= w;
+        for (int t = 0; ok && t < w; ++t) ok = col[rp[s0] + t] == s0 + t;
+        const int r = ok ? len0 - w : 0;
+        for (int d = 1; ok && d < w; ++d) {
+            const int row = s0 + d, a = rp[row];
+            ok = rp[row + 1] - a == (w - d) + r;
+            for (int t = 0; ok && t < w - d; ++t) ok = col[a + t] == row + t;
+            for (int q = 0; ok && q < r; ++q) ok = col[a + (w - d) + q] == col[rp[s0] + w + q];
+        }
+        if (!ok)
+            return fail(SPTRSV_ERR_BAD_INPUT, "rows " + std::to_string(s0) + ".." + std::to_string(s0 + w - 1) +
+                                                  " do not form a supernode (dense triangle + common columns)");
+        rcount[k] = r;
+        offt[k] = tot_t;
+        offr[k] = tot_r;
+        offc[k] = (int)cols.size();
+        tot_t += (int64_t)w * w;
+        tot_r += (int64_t)w * r;
+        int last = -1;
+        for (int q = 0; q < r; ++q) {
+            const int c = col[rp[s0] + w + q];
+            cols.push_back(c);
+            if (sn_of[(size_t)c] != last) {
+                last = sn_of[(size_t)c];
+                dep.push_back(last);
+            }
+        }
+        dep_ptr[k + 1] = (int)dep.size();
+    }
+    std::vector<double> T((size_t)tot_t, 0.0), R((size_t)tot_r);
+    for (int k = 0; k < S; ++k) {
+        const int s0 = sb[k], w = sb[k + 1] - s0, r = rcount[k];
+        for (int d = 0; d < w; ++d) {
+            const int a = rp[s0 + d];
+            for (int e = d; e < w; ++e) T[(size_t)(offt[k] + (int64_t)d * w + e)] = val[(size_t)(a + e - d)];
+            for (int q = 0; q < r; ++q) R[(size_t)(offr[k] + (int64_t)d * r + q)] = val[(size_t)(a + (w - d) + q)];
+        }
+    }
+
+    supernodal_free(P);
+    std::vector<int> flags(S, 0);
+    double* d_T = nullptr;
+    SPT_CHECK(upload(&P->sn_start, sb));
+    SPT_CHECK(upload(&P->sn_r, rcount));
+    SPT_CHECK(upload(&P->sn_offt, offt));
+    SPT_CHECK(upload(&P->sn_offr, offr));
+    SPT_CHECK(upload(&P->sn_offc, offc));
+    SPT_CHECK(upload(&P->sn_cols, cols));
+    SPT_CHECK(upload(&P->sn_rect, R));
+    SPT_CHECK(upload(&P->sn_dep_ptr, dep_ptr));
+    SPT_CHECK(upload(&P->sn_dep, dep));
+    SPT_CHECK(upload(&P->sn_flag, flags));
+    SPT_CHECK(upload(&d_T, T));
+    SPT_HIP(hipMalloc((void**)&P->sn_tinv, std::max<size_t>((size_t)tot_t, 1) * sizeof(double)));
+    k_sn_invert<<<S, 128, 0, P->stream>>>(P->sn_start, P->sn_offt, d_T, P->sn_tinv);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+    (void)hipFree(d_T);
+    if (e != hipSuccess) return fail(SPTRSV_ERR_HIP, std::string("supernode inversion: ") + hipGetErrorString(e));
+    P->sn = S;
+    P->sn_wmax = wmax;
+    P->sn_epoch = 0;
+    P->h_snodes = sb;
+    return SPTRSV_OK;
+}
+
+extern "C" int sptrsv_plan_get_supernodes(sptrsv_plan_t P, int* sn, int* snodes) {
+    if (!P) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
+    if (sn) *sn = P->sn;
+    if (snodes)
+        for (size_t i = 0; i < P->h_snodes.size(); ++i) snodes[i] = P->h_snodes[i];
+    return SPTRSV_OK;
+}

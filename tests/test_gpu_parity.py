@@ -19,6 +19,7 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 FP32_TOL = 2e-5
 EXACT = [capi.VAR_ROW_PULL, capi.VAR_LEVEL_PULL, capi.VAR_LEVELSET]
+EXACT_M1 = EXACT + [capi.VAR_TILE]  # the tile kernel solves one right-hand side
 
 
 def banner(msg):
@@ -98,7 +99,7 @@ def test_device_transpose_general_pattern():
     assert np.array_equal(cp, C.indptr) and np.array_equal(ri, C.indices) and np.array_equal(cv, C.data)
 
 
-@pytest.mark.parametrize("variant", EXACT + [capi.VAR_CSC_PUSH])
+@pytest.mark.parametrize("variant", EXACT_M1 + [capi.VAR_CSC_PUSH])
 @pytest.mark.parametrize("name", list(MATS))
 def test_fp64_single_rhs(plans, name, variant):
     U, P = MATS[name], plans[name]
@@ -106,7 +107,7 @@ def test_fp64_single_rhs(plans, name, variant):
     xt, b = gen.rhs_for(U, 1, seed=2)
     x = P.solve_host(b, capi.params(variant=variant))
     ref = oracle_solve(U, b)
-    if variant in EXACT:
+    if variant in EXACT_M1:
         assert np.array_equal(x, ref), O.rel_error(ref, x)
     else:
         assert O.rel_error(ref, x) <= FP64_TOL
@@ -208,7 +209,7 @@ def test_full_size_ic0_128():
         assert np.array_equal(P.indegree(), (1 + (x < N - 1) + (y < N - 1) + (z < N - 1)).astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=1)
         ref = oracle_solve(U, b)
-        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL):
+        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_TILE):
             banner(f"solve ic0_128 {capi.VARIANT_NAMES[variant]}")
             got = P.solve_host(b, capi.params(variant=variant))
             assert np.array_equal(got, ref)
@@ -225,8 +226,19 @@ def test_full_size_zipf_1e6_level_pull():
         assert P.n_levels == L
         assert np.array_equal(P.levels(), lv.astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=1)
-        got = P.solve_host(b, capi.params(variant=capi.VAR_ROW_PULL))
-        assert np.array_equal(got, oracle_solve(U, b))
+        ref = oracle_solve(U, b)
+        for variant in (capi.VAR_ROW_PULL, capi.VAR_TILE):
+            banner(f"solve zipf_1e6 {capi.VARIANT_NAMES[variant]}")
+            got = P.solve_host(b, capi.params(variant=variant))
+            assert np.array_equal(got, ref)
+
+
+def test_tile_rejects_multi_rhs(plans):
+    U, P = MATS["zipf_3000"], plans["zipf_3000"]
+    xt, b = gen.rhs_for(U, 2, seed=1, layout="row")
+    with pytest.raises(capi.SptrsvError) as e:
+        P.solve_host(b, capi.params(variant=capi.VAR_TILE))
+    assert e.value.code == capi.ERR_UNSUPPORTED
 
 
 def _build(src, out):
@@ -292,3 +304,77 @@ def test_cpp_dropin_entry_points(tmp_path, which, rhs):
         lo, _ = O.base_gauss_lower(cp.astype(np.uint64), ri, cv, b[0])
         assert O.rel_error(lo, load("lower")) <= FP64_TOL
     assert "nonsquare -1.0" in out.stdout
+
+
+# ---- supernodal (MFMA) ------------------------------------------------------------------------
+
+@pytest.mark.parametrize("m", [1, 8, 16, 32, 100, 128])
+def test_supernodal_given_partition(m):
+    U = gen.random_supernodal(24, seed=6)
+    banner(f"supernodal given m={m}")
+    with capi.Plan.from_upper(U) as P:
+        sn = P.build_supernodes(U.snodes)
+        assert np.array_equal(sn, U.snodes)  # widths <= 128 stay unsplit
+        xt, b = gen.rhs_for(U, m, seed=2, layout="row")
+        x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+        ref = oracle_solve(U, b)
+        assert O.rel_error(ref, x) <= FP64_TOL
+        bb = b if m == 1 else np.ascontiguousarray(b.T)
+        xs, _ = O.supernodal_upper(U.rowptr, U.col, U.val, U.snodes, bb)
+        assert O.rel_error(xs if m == 1 else xs.T, x) <= FP64_TOL
+
+
+def test_supernodal_detected_and_split():
+    U = gen.random_supernodal(16, seed=8, wmin=100, wmax=300)  # wider than 128 -> split
+    with capi.Plan.from_upper(U) as P:
+        sn = P.build_supernodes()
+        assert np.all(np.diff(sn) <= 128)
+        # detection may merge neighbouring generated supernodes whose patterns nest, but it
+        # never cuts a generated supernode except to keep widths <= 128
+        gen_b = set(U.snodes.tolist())
+        for k in range(len(U.snodes) - 1):
+            inside = [s for s in sn.tolist() if U.snodes[k] < s < U.snodes[k + 1]]
+            assert all((s - U.snodes[k]) % 128 == 0 or s in gen_b for s in inside) or \
+                all(np.diff([U.snodes[k]] + inside + [U.snodes[k + 1]]) <= 128)
+        xt, b = gen.rhs_for(U, 24, seed=3, layout="row")
+        x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
+def test_supernodal_on_general_pattern_and_fixture(plans, csr6, snodes6):
+    for name in ("zipf_3000", "chain_4000", "chol2d_15"):
+        U, P = MATS[name], plans[name]
+        P.build_supernodes()
+        xt, b = gen.rhs_for(U, 4, seed=4, layout="row")
+        x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, name
+    n, rp, col, val = csr6
+    with capi.Plan.from_csr(rp, col, val) as P:
+        P.build_supernodes(snodes6)
+        x = P.solve_host(np.ones(n), capi.params(variant=capi.VAR_SUPERNODAL))
+        np.testing.assert_allclose(x, [0.5, 7 / 8, 1 / 12, 0.5, 5 / 24, 1 / 6], rtol=0, atol=1e-15)
+
+
+def test_supernodal_rejects_bad_partition(plans):
+    P = plans["zipf_3000"]
+    with pytest.raises(capi.SptrsvError) as e:
+        P.build_supernodes(np.array([0, 7, 3000], dtype=np.int32))
+    assert e.value.code == capi.ERR_BAD_INPUT
+    with pytest.raises(capi.SptrsvError):
+        P.build_supernodes(np.array([0, 3001], dtype=np.int32))
+
+
+def test_supernodal_full_size_4096():
+    """configs: random supernodal factor, 4096 supernodes, widths U[16,128]."""
+    U = gen.random_supernodal(4096, seed=2)
+    banner(f"plan supernodal_4096 n={U.n} nnz={U.nnz}")
+    with capi.Plan.from_upper(U) as P:
+        P.build_supernodes(U.snodes)
+        for m in (8,):
+            xt, b = gen.rhs_for(U, m, seed=2, layout="row")
+            banner(f"solve supernodal_4096 m={m}")
+            x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+            ref = oracle_solve(U, b)
+            assert O.rel_error(ref, x) <= FP64_TOL
+            xl = P.solve_host(b, capi.params(variant=capi.VAR_LEVEL_PULL))
+            assert np.array_equal(xl, ref)

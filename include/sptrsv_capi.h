@@ -58,6 +58,8 @@ extern "C" {
 /* Value precision of the solve. */
 #define SPTRSV_FP64 0
 #define SPTRSV_FP32 1
+#define SPTRSV_BF16 2  /* supernodal variant only: BF16 rectangular products on MFMA with FP32
+                          accumulation and a power-of-two scale per supernode block; FP32 triangle */
 
 typedef struct {
     int variant;      /* SPTRSV_VAR_* */
@@ -140,6 +142,20 @@ int sptrsv_plan_timing_read(sptrsv_plan_t plan, double* total_ms, int* launches)
 
 /* Residual r = b - U x on the device, FP64, row-major n x m. */
 int sptrsv_residual(sptrsv_plan_t plan, const double* d_b, const double* d_x, double* d_r, int m);
+
+/* Mixed precision with FP64 iterative refinement (no reference counterpart).
+ * x_0 = solve_low(b); then repeat  r = b - U x (FP64 residual on the GPU),  d = solve_low(r),
+ * x += d (FP64)  until  max_k ||d_k||_inf / ||x_k||_inf <= tol  or max_iters corrections.
+ * solve_low is the kernel chosen by params->variant in opts->precision (FP32; BF16 only with
+ * the supernodal variant).  Synchronous; *iters = corrections applied, *ratio = last ratio.
+ * Reaching max_iters is not an error: check *ratio against tol. */
+typedef struct {
+    int max_iters;    /* 0 -> 30 */
+    int precision;    /* SPTRSV_FP32 or SPTRSV_BF16 */
+    double tol;       /* 0 -> 1e-14 */
+} sptrsv_refine_t;
+int sptrsv_solve_refined(sptrsv_plan_t plan, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
+                         const double* d_b, double* d_x, int m, int* iters, double* ratio);
 
 #ifdef __cplusplus
 }

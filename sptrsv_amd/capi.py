@@ -36,12 +36,17 @@ VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_P
 
 FP64 = 0
 FP32 = 1
+BF16 = 2
 
 
 class SptrsvError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str):
         super().__init__(f"{where}: status {code}: {msg}")
         self.code = code
+
+
+class RefineOpts(ctypes.Structure):
+    _fields_ = [("max_iters", ctypes.c_int), ("precision", ctypes.c_int), ("tol", ctypes.c_double)]
 
 
 class Params(ctypes.Structure):
@@ -97,6 +102,8 @@ def lib():
         "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
         "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
         "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
+        "sptrsv_solve_refined": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
+                                      ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_plan_get_supernodes": (_i, [_PLAN, ctypes.POINTER(_i), _vp]),
     }
     for name, (res, args) in sig.items():
@@ -314,6 +321,16 @@ class Plan:
         check(lib().sptrsv_solve_host(self.handle, ctypes.byref(prm or params()), _ptr(b), _ptr(x), int(m),
                                       int(bool(col_major))), "solve_host")
         return x
+
+    def solve_refined(self, d_b, d_x, m=1, prm=None, precision=FP32, tol=0.0, max_iters=0):
+        """Reduced-precision solves + FP64 refinement on the device (synchronous).
+        Returns (corrections applied, last max ||d||/||x||)."""
+        o = RefineOpts()
+        o.max_iters, o.precision, o.tol = max_iters, precision, tol
+        it, ratio = _i(0), ctypes.c_double(0.0)
+        check(lib().sptrsv_solve_refined(self.handle, ctypes.byref(prm or params()), ctypes.byref(o), d_b.ptr,
+                                         d_x.ptr, int(m), ctypes.byref(it), ctypes.byref(ratio)), "solve_refined")
+        return it.value, ratio.value
 
     def residual(self, d_b: DeviceArray, d_x: DeviceArray, d_r: DeviceArray, m: int = 1):
         check(lib().sptrsv_residual(self.handle, d_b.ptr, d_x.ptr, d_r.ptr, int(m)), "residual")

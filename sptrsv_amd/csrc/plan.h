@@ -70,6 +70,16 @@ struct sptrsv_plan_s {
     int* sn_flag = nullptr;      // sn, epoch of the last solve that published the supernode
     int sn_epoch = 0;
     std::vector<int> h_snodes;
+    std::vector<int> h_sn_r;        // rectangular column count per supernode
+    int64_t* sn_offr16 = nullptr;   // offsets of the BF16 blocks (rows padded to 16 columns)
+
+    // refinement work buffers (n*m)
+    double* ref_r = nullptr;
+    float* ref_r32 = nullptr;
+    float* ref_d32 = nullptr;
+    unsigned long long* ref_norms = nullptr;  // 2*m: max|d_k|, max|x_k| as ordered bit patterns
+    size_t ref_elems = 0;
+    size_t ref_cols = 0;
 
     // tiles for the TILE variant (built on first use): tile t covers rows [tile_lo[t],
     // tile_hi[t]) (tile 0 = last rows); tile_ext[tile_ext_ptr[t]..] are the columns outside the
@@ -96,6 +106,8 @@ int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, cons
 int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
                      uint64_t limit);  // supernodal.hip
 void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip
+int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
+                         uint64_t limit);  // supernodal.hip
 int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
                uint64_t limit);  // tile.hip
 void tile_free(sptrsv_plan_s* p);         // tile.hip

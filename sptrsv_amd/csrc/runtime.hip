@@ -218,7 +218,8 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     tile_free(p);
     void* bufs[] = {p->rowptr, p->col, p->val, p->val32, p->colptr, p->rowidx, p->cval, p->cval32,
                     p->indeg, p->levels, p->level_ptr, p->level_rows, p->lrowptr, p->lcol,
-                    p->lval, p->lval32, p->ticket, p->indeg_work, p->left_sum, p->b32, p->x32};
+                    p->lval, p->lval32, p->ticket, p->indeg_work, p->left_sum, p->b32, p->x32,
+                    p->ref_r, p->ref_r32, p->ref_d32, p->ref_norms};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& e : p->events) {

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "plan.h"
@@ -326,6 +327,56 @@ __global__ __launch_bounds__(256) void k_residual(int n, int m, const int* __res
     }
 }
 
+
+// ---- FP64 iterative refinement around reduced-precision solves ------------------------------
+
+// r32 = (float)(b - U x): FP64 products summed in ascending order, rounded once to FP32
+__global__ __launch_bounds__(256) void k_residual32(int n, int m, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    float* __restrict__ r32) {
+    const int64_t total = (int64_t)n * m;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int row = (int)(q / m), k = (int)(q % m);
+        double acc = 0.0;
+        for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
+        r32[q] = (float)(b[q] - acc);
+    }
+}
+
+// x += d (FP64) and per-column max|d|, max|x| (non-negative doubles order like their bits).
+// grid.y covers columns in groups of 64; 4 row-waves per block reduce through LDS.
+__global__ __launch_bounds__(256) void k_update(int n, int m, const float* __restrict__ d, double* __restrict__ x,
+                                                unsigned long long* __restrict__ norms, int first) {
+    __shared__ double sd[4][64], sx[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = blockIdx.y * 64 + lane;
+    double md = 0.0, mx = 0.0;
+    if (k < m) {
+        for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
+            const size_t at = (size_t)row * m + k;
+            const double dv = (double)d[at];
+            const double xv = first ? dv : x[at] + dv;
+            x[at] = xv;
+            md = fmax(md, fabs(dv));
+            mx = fmax(mx, fabs(xv));
+        }
+    }
+    sd[wv][lane] = md;
+    sx[wv][lane] = mx;
+    __syncthreads();
+    if (wv == 0 && k < m) {
+        md = fmax(fmax(sd[0][lane], sd[1][lane]), fmax(sd[2][lane], sd[3][lane]));
+        mx = fmax(fmax(sx[0][lane], sx[1][lane]), fmax(sx[2][lane], sx[3][lane]));
+        atomicMax(&norms[k], __builtin_bit_cast(unsigned long long, md));
+        atomicMax(&norms[m + k], __builtin_bit_cast(unsigned long long, mx));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0ull;
+}
+
 inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
     int64_t g = (count + block - 1) / block;
     if (g < 1) g = 1;
@@ -566,6 +617,79 @@ int sptrsv_solve_host(sptrsv_plan_t P, const sptrsv_params_t* params, const doub
     (void)hipFree(db);
     (void)hipFree(dx);
     return rc;
+}
+
+
+int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
+                         const double* d_b, double* d_x, int m, int* iters, double* ratio) {
+    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    sptrsv_params_t prm{};
+    if (params) prm = *params;
+    sptrsv_refine_t ro{};
+    if (opts) ro = *opts;
+    const int max_iters = ro.max_iters > 0 ? ro.max_iters : 30;
+    const double tol = ro.tol > 0 ? ro.tol : 1e-14;
+    const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
+    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
+    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_WINDOW)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile variant has no reduced-precision form");
+    if (variant == SPTRSV_VAR_SUPERNODAL ? low != SPTRSV_BF16 : low != SPTRSV_FP32)
+        return fail(SPTRSV_ERR_UNSUPPORTED,
+                    "refinement: BF16 inner solves need the supernodal variant, the others run FP32");
+    const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
+                                               : P->ticks_per_ms * 20000ull;
+    const int unroll = prm.unroll > 0 ? prm.unroll : 4;
+    const int n = P->n;
+    const int64_t nm = (int64_t)n * m;
+    hipStream_t s = P->stream;
+    if ((int64_t)P->ref_elems < nm || (int)P->ref_cols < m) {
+        if (P->ref_r32) SPT_HIP(hipFree(P->ref_r32));
+        if (P->ref_d32) SPT_HIP(hipFree(P->ref_d32));
+        if (P->ref_norms) SPT_HIP(hipFree(P->ref_norms));
+        P->ref_r32 = P->ref_d32 = nullptr;
+        P->ref_norms = nullptr;
+        SPT_HIP(hipMalloc((void**)&P->ref_r32, nm * sizeof(float)));
+        SPT_HIP(hipMalloc((void**)&P->ref_d32, nm * sizeof(float)));
+        SPT_HIP(hipMalloc((void**)&P->ref_norms, 2 * (size_t)m * sizeof(unsigned long long)));
+        P->ref_elems = (size_t)nm;
+        P->ref_cols = (size_t)m;
+    }
+    if (variant != SPTRSV_VAR_SUPERNODAL) SPT_CHECK(ensure_fp32(P));
+    auto solve_low = [&](const float* in, float* out) -> int {
+        if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve_low(P, low, in, out, m, limit);
+        return solve_typed<float>(P, variant, prm.group, unroll, in, out, m, P->val32, P->cval32, P->lval32, limit);
+    };
+    const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));
+    std::vector<unsigned long long> h((size_t)2 * m);
+    k_convert<double, float><<<grid_for(nm), 256, 0, s>>>(d_b, P->ref_r32, nm);
+    SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
+    k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
+    k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 1);
+    SPT_HIP(hipGetLastError());
+    SPT_CHECK(sptrsv_plan_sync(P));
+    double last = 1.0;
+    int done = 0;
+    for (int it = 1; it <= max_iters; ++it) {
+        k_residual32<<<grid_for(nm), 256, 0, s>>>(n, m, P->rowptr, P->col, P->val, d_x, d_b, P->ref_r32);
+        SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
+        k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
+        k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 0);
+        SPT_HIP(hipGetLastError());
+        SPT_CHECK(sptrsv_plan_sync(P));
+        SPT_HIP(hipMemcpy(h.data(), P->ref_norms, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        last = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const double dn = __builtin_bit_cast(double, h[(size_t)k]);
+            const double xn = __builtin_bit_cast(double, h[(size_t)m + k]);
+            last = std::max(last, xn > 0 ? dn / xn : dn);
+        }
+        done = it;
+        if (!(last > tol)) break;
+    }
+    if (iters) *iters = done;
+    if (ratio) *ratio = last;
+    return SPTRSV_OK;
 }
 
 int sptrsv_residual(sptrsv_plan_t P, const double* d_b, const double* d_x, double* d_r, int m) {

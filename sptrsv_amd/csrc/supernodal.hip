@@ -173,6 +173,197 @@ int upload(X** dst, const std::vector<X>& src) {
     return SPTRSV_OK;
 }
 
+
+// ---- reduced precision (inner solves of the FP64 refinement) ------------------------------
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x16bf16_1k (measured): A(i,k) is element k%4 of lane i + 16*(k/4), B(k,j) is
+// element k%4 of lane j + 16*(k/4); result element v of lane l is D(4*(l/16) + v, l%16).
+// v_mfma_f32_16x16x4f32: A(i,k) lane i + 16k, B(k,j) lane j + 16k, D like the bf16 form.
+
+__device__ __forceinline__ short to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (values are finite)
+    return (short)(u >> 16);
+}
+
+// one block per supernode: scale = 2^ceil(log2 max|R|), R16 = bf16(R / scale), rows padded to
+// a multiple of 16 columns (zeros) so every MFMA k-step reads whole aligned groups
+__global__ void k_sn_pack16(const int* __restrict__ start, const int* __restrict__ rcount,
+                            const int64_t* __restrict__ offr, const int64_t* __restrict__ offr16,
+                            const double* __restrict__ R, uint16_t* __restrict__ R16, float* __restrict__ rscale) {
+    __shared__ double red[256];
+    const int k = blockIdx.x;
+    const int w = start[k + 1] - start[k], r = rcount[k], rp = (r + 15) & ~15;
+    const double* src = R + offr[k];
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < w * r; q += blockDim.x) mx = fmax(mx, fabs(src[q]));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double scale = red[0] > 0.0 ? exp2(ceil(log2(red[0]))) : 1.0;
+    if (threadIdx.x == 0) rscale[k] = (float)scale;
+    uint16_t* dst = R16 + offr16[k];
+    for (int q = threadIdx.x; q < w * rp; q += blockDim.x) {
+        const int d = q / rp, c = q % rp;
+        dst[q] = c < r ? (uint16_t)to_bf16((float)(src[(size_t)d * r + c] / scale)) : (uint16_t)0;
+    }
+}
+
+__global__ void k_sn_to_float(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+
+struct SnLowArgs {
+    int sn;
+    int m;
+    int bf16;  // rectangular product in BF16 (else FP32 on the VALU)
+    const int* start;
+    const int* rcount;
+    const int64_t* offt;
+    const int64_t* offr;
+    const int64_t* offr16;
+    const int* offc;
+    const int* cols;
+    const float* tinv;
+    const double* rect;
+    const uint16_t* rect16;
+    const float* rscale;
+    const int* dep_ptr;
+    const int* dep;
+    int* flag;
+    int epoch;
+    const float* b;
+    float* x;
+    int* ctrl;
+    uint64_t limit;
+};
+
+template <int MC>
+__global__ __launch_bounds__(256) void k_sn_solve_low(SnLowArgs a) {
+    __shared__ float W[kMaxW * MC];
+    __shared__ int s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
+    __syncthreads();
+    const int k = a.sn - 1 - s_ticket;
+    if (k < 0) return;
+    const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
+    if (tid == 0) {
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        int idle = 0;
+        bool give_up = false;
+        for (int q = a.dep_ptr[k]; q < a.dep_ptr[k + 1] && !give_up; ++q) {
+            const int d = a.dep[q];
+            while (__hip_atomic_load(&a.flag[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if ((++idle & 63) == 0 && wd.expired()) {
+                    give_up = true;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    const int nI = (w + 15) >> 4;
+    constexpr int nJ = MC / 16;
+    const int tiles = nI * nJ;
+    const int c16 = lane & 15, quad = lane >> 4;
+    const int rp = (r + 15) & ~15;
+    const uint16_t* R16 = a.rect16 + a.offr16[k];
+    const float* Ti = a.tinv + a.offt[k];
+    const int* C = a.cols + a.offc[k];
+    const float sc = a.rscale[k];
+    const size_t m = (size_t)a.m;
+    for (int j0 = 0; j0 < a.m; j0 += MC) {
+        // W = B_s - scale * (R16 * bf16(X(rect, :)))
+        for (int t = wave; t < tiles; t += 4) {
+            const int I = t / nJ, J = t % nJ;
+            const int jj = j0 + 16 * J + c16;
+            const bool jv = jj < a.m;
+            const int ia = 16 * I + c16;
+            const bool iav = ia < w;
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            const v4s zero = {0, 0, 0, 0};
+            for (int k0 = 0; k0 < rp; k0 += 16) {
+                const int kb = k0 + 4 * quad;
+                const v4s av = iav ? *reinterpret_cast<const v4s*>(R16 + (size_t)ia * rp + kb) : zero;
+                v4s bvv;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int kq = kb + q;
+                    const float xv = (kq < r && jv) ? a.x[(size_t)C[kq] * m + jj] : 0.f;
+                    bvv[q] = to_bf16(xv);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bvv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * I + 4 * quad + v;
+                W[i * MC + 16 * J + c16] = (i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] - sc * acc[v] : 0.f;
+            }
+        }
+        __syncthreads();
+        // X_s = Tinv_s * W in FP32 on MFMA
+        for (int t = wave; t < tiles; t += 4) {
+            const int I = t / nJ, J = t % nJ;
+            const int jj = j0 + 16 * J + c16;
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            const int ia = 16 * I + c16;
+            const bool iav = ia < w;
+            const float* Trow = Ti + (size_t)(iav ? ia : 0) * w;
+#pragma unroll 4
+            for (int k0 = 16 * I; k0 < w; k0 += 4) {
+                const int kk = k0 + quad;
+                const float av = (iav && kk < w) ? Trow[kk] : 0.f;
+                const float bvv = W[kk * MC + 16 * J + c16];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bvv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * I + 4 * quad + v;
+                if (i < w && jj < a.m) a.x[(size_t)(s0 + i) * m + jj] = acc[v];
+            }
+        }
+        __syncthreads();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int ensure_sn_low(sptrsv_plan_s* P) {
+    if (P->sn_rect16) return SPTRSV_OK;
+    const int S = P->sn;
+    std::vector<int64_t> off16(S);
+    int64_t tot16 = 0, tot_t = 0;
+    for (int k = 0; k < S; ++k) {
+        const int w = P->h_snodes[k + 1] - P->h_snodes[k];
+        off16[k] = tot16;
+        tot16 += (int64_t)w * ((P->h_sn_r[k] + 15) & ~15);
+        tot_t += (int64_t)w * w;
+    }
+    SPT_CHECK(upload(&P->sn_offr16, off16));
+    SPT_HIP(hipMalloc((void**)&P->sn_rect16, std::max<int64_t>(tot16, 1) * sizeof(uint16_t)));
+    SPT_HIP(hipMalloc((void**)&P->sn_rscale, (size_t)S * sizeof(float)));
+    SPT_HIP(hipMalloc((void**)&P->sn_tinv32, std::max<int64_t>(tot_t, 1) * sizeof(float)));
+    k_sn_pack16<<<S, 256, 0, P->stream>>>(P->sn_start, P->sn_r, P->sn_offr, P->sn_offr16, P->sn_rect,
+                                          P->sn_rect16, P->sn_rscale);
+    k_sn_to_float<<<(unsigned)std::min<int64_t>((tot_t + 255) / 256, 65536), 256, 0, P->stream>>>(
+        P->sn_tinv, P->sn_tinv32, tot_t);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
 }  // namespace
 
 void supernodal_free(sptrsv_plan_s* p) {
@@ -186,6 +377,9 @@ void supernodal_free(sptrsv_plan_s* p) {
     p->sn_tinv32 = nullptr;
     p->sn_rect16 = nullptr;
     p->sn_rscale = nullptr;
+    if (p->sn_offr16) (void)hipFree(p->sn_offr16);
+    p->sn_offr16 = nullptr;
+    p->h_sn_r.clear();
     p->sn = 0;
     p->h_snodes.clear();
 }
@@ -204,6 +398,27 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     if (m <= 16) k_sn_solve<16><<<P->sn, 256, 0, P->stream>>>(a);
     else if (m <= 32) k_sn_solve<32><<<P->sn, 256, 0, P->stream>>>(a);
     else k_sn_solve<64><<<P->sn, 256, 0, P->stream>>>(a);
+    end_timed(P, stop);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int supernodal_solve_low(sptrsv_plan_s* P, int precision, const float* d_b32, float* d_x32, int m, uint64_t limit) {
+    if (P->sn == 0)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
+    if (precision != SPTRSV_BF16)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "reduced-precision supernodal solve is BF16 (rectangular) + FP32");
+    SPT_CHECK(ensure_sn_low(P));
+    P->sn_epoch++;
+    SnLowArgs a{P->sn, m, precision == SPTRSV_BF16, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offr16,
+                P->sn_offc, P->sn_cols, P->sn_tinv32, P->sn_rect, P->sn_rect16, P->sn_rscale, P->sn_dep_ptr,
+                P->sn_dep, P->sn_flag, P->sn_epoch, d_b32, d_x32, P->ticket, limit};
+    k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
+    hipEvent_t stop;
+    SPT_CHECK(begin_timed(P, &stop));
+    if (m <= 16) k_sn_solve_low<16><<<P->sn, 256, 0, P->stream>>>(a);
+    else if (m <= 32) k_sn_solve_low<32><<<P->sn, 256, 0, P->stream>>>(a);
+    else k_sn_solve_low<64><<<P->sn, 256, 0, P->stream>>>(a);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
@@ -324,6 +539,7 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
     (void)hipFree(d_T);
     if (e != hipSuccess) return fail(SPTRSV_ERR_HIP, std::string("supernode inversion: ") + hipGetErrorString(e));
     P->sn = S;
+    P->h_sn_r = rcount;
     P->sn_wmax = wmax;
     P->sn_epoch = 0;
     P->h_snodes = sb;

@@ -153,6 +153,8 @@ __global__ __launch_bounds__(kTileBlock) void k_tile(TileArgs a) {
         }
         if (__all(done)) break;
         if (!__any(progressed)) {
+            // an idle wave backs off so the waves on the dependency chain keep the SIMD and LDS
+            __builtin_amdgcn_s_sleep(1);
             if ((++idle & 255) == 0 && !done && wd.expired()) done = true;
         }
     }

@@ -378,3 +378,59 @@ def test_supernodal_full_size_4096():
             assert O.rel_error(ref, x) <= FP64_TOL
             xl = P.solve_host(b, capi.params(variant=capi.VAR_LEVEL_PULL))
             assert np.array_equal(xl, ref)
+
+
+# ---- mixed precision with FP64 refinement -----------------------------------------------------
+
+def _refined(P, b, m, variant, precision, tol=0.0, max_iters=0):
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    it, ratio = P.solve_refined(d_b, d_x, m, capi.params(variant=variant), precision=precision, tol=tol,
+                                max_iters=max_iters)
+    return d_x.download(), it, ratio
+
+
+@pytest.mark.parametrize("variant", [capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET, capi.VAR_CSC_PUSH])
+@pytest.mark.parametrize("m", [1, 8])
+def test_fp32_refined_reaches_fp64(plans, variant, m):
+    for name in ("zipf_3000", "ic0_12", "chol2d_15", "snode_10"):
+        U, P = MATS[name], plans[name]
+        banner(f"refine fp32 {name} m={m} {capi.VARIANT_NAMES[variant]}")
+        xt, b = gen.rhs_for(U, m, seed=10, layout="row")
+        x, it, ratio = _refined(P, b, m, variant, capi.FP32)
+        assert ratio <= 1e-14 and 1 <= it <= 8, (name, it, ratio)
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, name
+
+
+@pytest.mark.parametrize("m", [1, 8, 32, 128])
+def test_bf16_supernodal_refined_reaches_fp64(m):
+    U = gen.random_supernodal(24, seed=6)
+    with capi.Plan.from_upper(U) as P:
+        P.build_supernodes(U.snodes)
+        xt, b = gen.rhs_for(U, m, seed=11, layout="row")
+        x, it, ratio = _refined(P, b, m, capi.VAR_SUPERNODAL, capi.BF16)
+        assert ratio <= 1e-14, (it, ratio)
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
+def test_refinement_reports_non_convergence():
+    """diag_scale << 1 removes the diagonal dominance: FP32 solves cannot contract the error,
+    and the driver must say so through the returned ratio instead of claiming convergence."""
+    U = gen.random_zipf(2000, seed=12, diag_scale=0.02)
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 1, seed=1)
+        x, it, ratio = _refined(P, b, 1, capi.VAR_LEVEL_PULL, capi.FP32, max_iters=3)
+        ref = oracle_solve(U, b)
+        if ratio > 1e-14:
+            assert it == 3
+        else:
+            assert O.rel_error(ref, x) <= FP64_TOL
+
+
+def test_refinement_rejects_unsupported(plans):
+    U, P = MATS["zipf_3000"], plans["zipf_3000"]
+    xt, b = gen.rhs_for(U, 1, seed=1)
+    for variant, prec in ((capi.VAR_TILE, capi.FP32), (capi.VAR_LEVEL_PULL, capi.BF16)):
+        with pytest.raises(capi.SptrsvError) as e:
+            _refined(P, b, 1, variant, prec)
+        assert e.value.code == capi.ERR_UNSUPPORTED

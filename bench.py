@@ -147,7 +147,7 @@ def main():
     if not args.no_extras:
         runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
                 ("ic0_128", "level_pull", 32), ("ic0_128", "level_pull", 128), ("zipf_1e6", "tile", 1),
-                ("zipf_1e6", "row_pull", 1), ("chol2d_100", "tile", 1), ("chol2d_100", "row_pull", 1)]
+                ("zipf_1e6", "chain", 1), ("chol2d_100", "chain", 1), ("chol2d_100", "level_pull", 1)]
         cache = {args.workload: (U, P)}
         for wl, vname, mm in runs:
             if (wl, vname, mm) == (args.workload, args.variant, m):
@@ -164,7 +164,16 @@ def main():
                 e_ms, e_kms = time_solves(capi, PP, db, dx, mm, capi.params(variant=names[vname]),
                                           3 if mm > 1 or wl != "ic0_128" else 10, 2, e_first)
                 BB = alg_bytes(UU.n, UU.nnz, mm)
+                xg = dx.download()
+                if mm == 1 and wl != "ic0_128":
+                    ref_w, t_cpu = O.base_gauss_upper(UU.rowptr, UU.col, UU.val, bx)
+                    cpu_ms = {f"{wl}/cpu_serial_ms": round(t_cpu * 1e3, 3)}
+                    extras.update(cpu_ms)
+                    e_err = O.rel_error(ref_w, xg)
+                else:
+                    e_err = None
                 extras[f"{wl}/{vname}/m{mm}"] = {"ms_per_step": round(e_ms, 4), "kernel_ms": round(e_kms, 4),
+                                                 "rel_err_vs_oracle": e_err,
                                                  "GB_s": round(BB / (e_ms / 1e3) / 1e9, 2),
                                                  "nnz_per_s": UU.nnz * mm / (e_ms / 1e3), "first_ms": e_first}
                 db.free()

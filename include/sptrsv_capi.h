@@ -44,8 +44,9 @@ extern "C" {
                                      through its own value; exact reference summation order */
 #define SPTRSV_VAR_LEVEL_PULL 3   /* same kernel over level-ordered rows (level-permuted CSR) */
 #define SPTRSV_VAR_LEVELSET 4     /* bulk-synchronous: one launch per level (src/core.cpp:273-376) */
-#define SPTRSV_VAR_WINDOW 5       /* wave per 64 consecutive rows, in-window dependencies resolved
-                                     through cross-lane broadcasts (chains) */
+#define SPTRSV_VAR_CHAIN 5        /* one wave walks 64-row windows in order: window tile in LDS,
+                                     64 broadcast steps per window, recent x in an LDS ring; for
+                                     factors whose dependency graph is (nearly) a chain; m == 1 */
 #define SPTRSV_VAR_SUPERNODAL 6   /* one workgroup per supernode: dense blocks on MFMA, the
                                      diagonal block applied through its precomputed inverse
                                      (needs sptrsv_plan_build_supernodes) */

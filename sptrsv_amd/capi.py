@@ -27,11 +27,11 @@ VAR_CSC_PUSH = 1
 VAR_ROW_PULL = 2
 VAR_LEVEL_PULL = 3
 VAR_LEVELSET = 4
-VAR_WINDOW = 5
+VAR_CHAIN = 5
 VAR_SUPERNODAL = 6
 VAR_TILE = 7
 VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
-                 VAR_LEVELSET: "levelset", VAR_WINDOW: "window",
+                 VAR_LEVELSET: "levelset", VAR_CHAIN: "chain",
                  VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile"}
 
 FP64 = 0

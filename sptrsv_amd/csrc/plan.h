@@ -111,6 +111,8 @@ int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, fl
 int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
                uint64_t limit);  // tile.hip
 void tile_free(sptrsv_plan_s* p);         // tile.hip
+int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
+int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);
 void end_timed(sptrsv_plan_s* p, hipEvent_t stop);
 }  // namespace sptrsv

@@ -549,13 +549,14 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     sptrsv_params_t prm{};
     if (params) prm = *params;
     int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
-    if (variant < 1 || variant >= SPTRSV_VAR_COUNT || variant == SPTRSV_VAR_WINDOW)
+    if (variant < 1 || variant >= SPTRSV_VAR_COUNT)
         return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
     const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
                                             : P->ticks_per_ms * 20000ull;
     const int unroll = prm.unroll > 0 ? prm.unroll : 4;
     if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, d_b, d_x, m, limit);
+    if (variant == SPTRSV_VAR_CHAIN) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, d_b, d_x, m, P->val, P->cval, P->lval, limit);
     if (prm.precision == SPTRSV_FP32) {
@@ -632,8 +633,10 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     const double tol = ro.tol > 0 ? ro.tol : 1e-14;
     const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
     int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
-    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_WINDOW)
+    if (variant == SPTRSV_VAR_TILE)
         return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile variant has no reduced-precision form");
+    if (variant == SPTRSV_VAR_CHAIN && m != 1)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the chain variant solves one right-hand side");
     if (variant == SPTRSV_VAR_SUPERNODAL ? low != SPTRSV_BF16 : low != SPTRSV_FP32)
         return fail(SPTRSV_ERR_UNSUPPORTED,
                     "refinement: BF16 inner solves need the supernodal variant, the others run FP32");
@@ -658,6 +661,7 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     if (variant != SPTRSV_VAR_SUPERNODAL) SPT_CHECK(ensure_fp32(P));
     auto solve_low = [&](const float* in, float* out) -> int {
         if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve_low(P, low, in, out, m, limit);
+        if (variant == SPTRSV_VAR_CHAIN) return chain_solve32(P, in, out);
         return solve_typed<float>(P, variant, prm.group, unroll, in, out, m, P->val32, P->cval32, P->lval32, limit);
     };
     const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));

@@ -113,6 +113,32 @@ def test_fp64_single_rhs(plans, name, variant):
         assert O.rel_error(ref, x) <= FP64_TOL
 
 
+@pytest.mark.parametrize("name", list(MATS))
+def test_fp64_chain(plans, name):
+    U, P = MATS[name], plans[name]
+    banner(f"solve chain {name}")
+    xt, b = gen.rhs_for(U, 1, seed=13)
+    x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
+    assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4096 + 130, 9000])
+def test_chain_window_and_ring_edges(n):
+    """window boundaries, a partial last window, and entries older than the 4096-row ring"""
+    rng = np.random.default_rng(n)
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [rng.uniform(2, 3, n)]
+    for off in (1, 2, 63, 64, 65, 4095, 4096, 4097, 5000):
+        r = np.arange(max(n - off, 0))
+        rows.append(r); cols.append(r + off); vals.append(rng.uniform(-0.2, 0.2, len(r)))
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    o = np.lexsort((cols, rows))
+    U = gen._from_rows(n, rows[o], cols[o], vals[o], "edges")
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 1, seed=1)
+        x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
+        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
 @pytest.mark.parametrize("group", [2, 8, 64])
 @pytest.mark.parametrize("name", ["zipf_3000", "chol2d_15", "dense_70", "chain_4000"])
 def test_fp64_entry_split(plans, name, group):
@@ -231,6 +257,8 @@ def test_full_size_zipf_1e6_level_pull():
             banner(f"solve zipf_1e6 {capi.VARIANT_NAMES[variant]}")
             got = P.solve_host(b, capi.params(variant=variant))
             assert np.array_equal(got, ref)
+        got = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
+        assert O.rel_error(ref, got) <= FP64_TOL
 
 
 def test_tile_rejects_multi_rhs(plans):
@@ -390,9 +418,12 @@ def _refined(P, b, m, variant, precision, tol=0.0, max_iters=0):
     return d_x.download(), it, ratio
 
 
-@pytest.mark.parametrize("variant", [capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET, capi.VAR_CSC_PUSH])
+@pytest.mark.parametrize("variant", [capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET, capi.VAR_CSC_PUSH,
+                                     capi.VAR_CHAIN])
 @pytest.mark.parametrize("m", [1, 8])
 def test_fp32_refined_reaches_fp64(plans, variant, m):
+    if variant == capi.VAR_CHAIN and m > 1:
+        pytest.skip("the chain kernel solves one right-hand side")
     for name in ("zipf_3000", "ic0_12", "chol2d_15", "snode_10"):
         U, P = MATS[name], plans[name]
         banner(f"refine fp32 {name} m={m} {capi.VARIANT_NAMES[variant]}")

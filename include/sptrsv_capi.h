@@ -158,6 +158,51 @@ typedef struct {
 int sptrsv_solve_refined(sptrsv_plan_t plan, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
                          const double* d_b, double* d_x, int m, int* iters, double* ratio);
 
+/* ---- batches of independent systems (no reference counterpart) -------------------------
+ * The systems are given as one block-diagonal upper CSR: system s owns rows
+ * sys_ptr[s] .. sys_ptr[s+1]-1 and only references its own columns.  Every system is solved by
+ * one persistent workgroup with its own configuration (SPTRSV_BCFG_*), systems popped from a
+ * queue in a caller-given order; sptrsv_batch_solve can replay the launch as a captured
+ * hipGraph.  Device x/b are row-major (n_total x m). */
+#define SPTRSV_BCFG_LS1 0   /* workgroup level-set, 1 lane per row */
+#define SPTRSV_BCFG_LS4 1   /* 4 lanes per row (entries when m == 1, RHS columns otherwise) */
+#define SPTRSV_BCFG_LS16 2
+#define SPTRSV_BCFG_LS64 3
+#define SPTRSV_BCFG_CHAIN 4 /* one wave, 64-row windows (m == 1) */
+#define SPTRSV_BCFG_COUNT 5
+typedef struct sptrsv_batch_s* sptrsv_batch_t;
+int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t* rowptr, const int* col,
+                        const double* val, sptrsv_batch_t* batch);
+int sptrsv_batch_destroy(sptrsv_batch_t batch);
+int sptrsv_batch_info(sptrsv_batch_t batch, int* nsys, int* n, int64_t* nnz);
+/* 24 selector features per system (row-major nsys x 24), see DESIGN.md for the list. */
+int sptrsv_batch_features(sptrsv_batch_t batch, int m, float* feats);
+/* Per-system configuration and processing order (either may be NULL). */
+int sptrsv_batch_set_configs(sptrsv_batch_t batch, const int* cfg, const int* order);
+int sptrsv_batch_solve(sptrsv_batch_t batch, int precision, const double* d_b, double* d_x, int m, int use_graph);
+int sptrsv_batch_sync(sptrsv_batch_t batch);
+/* The block-diagonal plan under the batch (for residuals, timing, refinement). */
+sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t batch);
+/* Autotuner primitive: every system on `cfg` in one launch; device microseconds per system. */
+int sptrsv_batch_time_config(sptrsv_batch_t batch, int cfg, int precision, const double* d_b, double* d_x, int m,
+                             double* sys_us);
+
+/* ---- kernel-selector network (no reference counterpart) ---------------------------------
+ * in -> h1 -> h2 -> out, ReLU, masked softmax cross-entropy, FP32, forward/backward/Adam on the
+ * GPU.  Parameters: W1[in][h1], b1[h1], W2[h1][h2], b2[h2], W3[h2][out], b3[out]. */
+typedef struct sptrsv_mlp_s* sptrsv_mlp_t;
+int sptrsv_mlp_create(int in, int h1, int h2, int out, unsigned seed, sptrsv_mlp_t* mlp);
+int sptrsv_mlp_destroy(sptrsv_mlp_t mlp);
+int sptrsv_mlp_num_params(sptrsv_mlp_t mlp, int* count);
+int sptrsv_mlp_get_params(sptrsv_mlp_t mlp, float* host);
+int sptrsv_mlp_set_params(sptrsv_mlp_t mlp, const float* host);
+/* per-sample losses (B) and the gradient of the mean loss (nparams); mask may be NULL */
+int sptrsv_mlp_loss_grad(sptrsv_mlp_t mlp, const float* X, const int* y, const float* mask, int B, float* losses,
+                         float* grad);
+int sptrsv_mlp_forward(sptrsv_mlp_t mlp, const float* X, int B, float* logits);
+int sptrsv_mlp_train(sptrsv_mlp_t mlp, const float* X, const int* y, const float* mask, int B, int epochs, float lr,
+                     float* final_loss);
+
 #ifdef __cplusplus
 }
 #endif

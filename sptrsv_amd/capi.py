@@ -102,6 +102,23 @@ def lib():
         "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
         "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
         "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
+        "sptrsv_batch_create": (_i, [_i, _vp, _i64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+        "sptrsv_batch_destroy": (_i, [_vp]),
+        "sptrsv_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i64)]),
+        "sptrsv_batch_features": (_i, [_vp, _i, _vp]),
+        "sptrsv_batch_set_configs": (_i, [_vp, _vp, _vp]),
+        "sptrsv_batch_solve": (_i, [_vp, _i, _vp, _vp, _i, _i]),
+        "sptrsv_batch_sync": (_i, [_vp]),
+        "sptrsv_batch_plan": (_vp, [_vp]),
+        "sptrsv_batch_time_config": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp]),
+        "sptrsv_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_uint, ctypes.POINTER(_vp)]),
+        "sptrsv_mlp_destroy": (_i, [_vp]),
+        "sptrsv_mlp_num_params": (_i, [_vp, ctypes.POINTER(_i)]),
+        "sptrsv_mlp_get_params": (_i, [_vp, _vp]),
+        "sptrsv_mlp_set_params": (_i, [_vp, _vp]),
+        "sptrsv_mlp_loss_grad": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
+        "sptrsv_mlp_forward": (_i, [_vp, _vp, _i, _vp]),
+        "sptrsv_mlp_train": (_i, [_vp, _vp, _vp, _vp, _i, _i, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
         "sptrsv_solve_refined": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                       ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_plan_get_supernodes": (_i, [_PLAN, ctypes.POINTER(_i), _vp]),
@@ -215,6 +232,7 @@ class Plan:
 
     def __init__(self, handle):
         self.handle = handle
+        self._owned = True
         n, nnz, L = _i(0), _i64(0), _i(0)
         check(lib().sptrsv_plan_info(self.handle, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(L)), "info")
         self.n, self.nnz, self.n_levels = n.value, nnz.value, L.value
@@ -246,9 +264,9 @@ class Plan:
         return cls(h)
 
     def close(self):
-        if self.handle:
+        if self.handle and self._owned:
             lib().sptrsv_plan_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -342,3 +360,120 @@ class Plan:
         ms, cnt = ctypes.c_double(0.0), _i(0)
         check(lib().sptrsv_plan_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(cnt)), "timing_read")
         return ms.value, cnt.value
+
+
+BCFG_LS1, BCFG_LS4, BCFG_LS16, BCFG_LS64, BCFG_CHAIN = range(5)
+BCFG_COUNT = 5
+BCFG_NAMES = ["ls1", "ls4", "ls16", "ls64", "chain"]
+
+
+class Batch:
+    """Independent systems solved together (one persistent launch); see sptrsv_batch_*."""
+
+    def __init__(self, U, sys_ptr):
+        rp = np.ascontiguousarray(U.rowptr, dtype=np.int64)
+        c = np.ascontiguousarray(U.col, dtype=np.int32)
+        v = np.ascontiguousarray(U.val, dtype=np.float64)
+        sp = np.ascontiguousarray(sys_ptr, dtype=np.int32)
+        h = _vp()
+        check(lib().sptrsv_batch_create(len(sp) - 1, _ptr(sp), len(c), _ptr(rp), _ptr(c), _ptr(v), ctypes.byref(h)),
+              "batch_create")
+        self.handle = h
+        self.nsys = len(sp) - 1
+        self.n = U.n
+        self.plan = Plan(_PLAN(lib().sptrsv_batch_plan(h)))
+        self.plan._owned = False
+
+    def close(self):
+        if self.handle:
+            self.plan.handle = None  # owned by the batch
+            lib().sptrsv_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def features(self, m=1):
+        out = np.zeros((self.nsys, 24), dtype=np.float32)
+        check(lib().sptrsv_batch_features(self.handle, int(m), _ptr(out)), "batch_features")
+        return out
+
+    def set_configs(self, cfg=None, order=None):
+        c = None if cfg is None else np.ascontiguousarray(cfg, dtype=np.int32)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        check(lib().sptrsv_batch_set_configs(self.handle, None if c is None else _ptr(c),
+                                             None if o is None else _ptr(o)), "batch_set_configs")
+
+    def solve(self, d_b, d_x, m=1, precision=FP64, graph=False):
+        check(lib().sptrsv_batch_solve(self.handle, int(precision), d_b.ptr, d_x.ptr, int(m), int(bool(graph))),
+              "batch_solve")
+
+    def sync(self):
+        check(lib().sptrsv_batch_sync(self.handle), "batch_sync")
+
+    def time_config(self, cfg, d_b, d_x, m=1, precision=FP64):
+        us = np.zeros(self.nsys, dtype=np.float64)
+        check(lib().sptrsv_batch_time_config(self.handle, int(cfg), int(precision), d_b.ptr, d_x.ptr, int(m), _ptr(us)),
+              "batch_time_config")
+        return us
+
+
+class MLP:
+    """The selector network on the GPU (sptrsv_mlp_*)."""
+
+    def __init__(self, nin=24, h1=64, h2=64, nout=5, seed=0):
+        h = _vp()
+        check(lib().sptrsv_mlp_create(nin, h1, h2, nout, seed, ctypes.byref(h)), "mlp_create")
+        self.handle = h
+        self.sizes = (nin, h1, h2, nout)
+        cnt = _i(0)
+        check(lib().sptrsv_mlp_num_params(h, ctypes.byref(cnt)), "mlp_num_params")
+        self.nparams = cnt.value
+
+    def close(self):
+        if self.handle:
+            lib().sptrsv_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        check(lib().sptrsv_mlp_get_params(self.handle, _ptr(p)), "mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().sptrsv_mlp_set_params(self.handle, _ptr(p)), "mlp_set_params")
+
+    def loss_grad(self, X, y, mask=None):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        losses = np.zeros(len(y), dtype=np.float32)
+        grad = np.zeros(self.nparams, dtype=np.float32)
+        check(lib().sptrsv_mlp_loss_grad(self.handle, _ptr(X), _ptr(y), None if mk is None else _ptr(mk), len(y),
+                                         _ptr(losses), _ptr(grad)), "mlp_loss_grad")
+        return losses, grad
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros((len(X), self.sizes[3]), dtype=np.float32)
+        check(lib().sptrsv_mlp_forward(self.handle, _ptr(X), len(X), _ptr(out)), "mlp_forward")
+        return out
+
+    def train(self, X, y, mask=None, epochs=500, lr=1e-2):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        fl = ctypes.c_float(0.0)
+        check(lib().sptrsv_mlp_train(self.handle, _ptr(X), _ptr(y), None if mk is None else _ptr(mk), len(y),
+                                     int(epochs), float(lr), ctypes.byref(fl)), "mlp_train")
+        return fl.value

@@ -113,6 +113,8 @@ int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, 
 void tile_free(sptrsv_plan_s* p);         // tile.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
+int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip
+int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s);  // solve.hip
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);
 void end_timed(sptrsv_plan_s* p, hipEvent_t stop);
 }  // namespace sptrsv

@@ -531,6 +531,18 @@ int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, const T* d
 
 }  // namespace
 
+int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s) {
+    k_convert<double, float><<<grid_for(count), 256, 0, s>>>(in, out, count);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
+    k_convert<float, double><<<grid_for(count), 256, 0, s>>>(in, out, count);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
 int default_variant(const sptrsv_plan_s* P, int m) {
     (void)m;
     (void)P;

@@ -223,6 +223,59 @@ def random_supernodal(S: int = 4096, seed: int = 2, wmin: int = 16, wmax: int = 
     return UpperCSR(n, rowptr, col, val, start.astype(np.int32), f"random_supernodal_{S}")
 
 
+def banded(n: int, bw: int, seed: int = 0) -> UpperCSR:
+    """Random banded upper factor: every row holds all columns i..min(i+bw, n-1); off-diagonal
+    values U(-1,1), diagonal 1 + sum |offdiag| of its row (diagonally dominant)."""
+    rng = np.random.default_rng(seed)
+    lens = np.minimum(bw + 1, n - np.arange(n))
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=rowptr[1:])
+    rows = np.repeat(np.arange(n), lens)
+    d = np.arange(rowptr[-1]) - np.repeat(rowptr[:-1], lens)
+    col = (rows + d).astype(np.int32)
+    val = rng.uniform(-1.0, 1.0, len(col))
+    off = d > 0
+    absum = np.bincount(rows[off], weights=np.abs(val[off]), minlength=n)
+    val[~off] = 1.0 + absum
+    return UpperCSR(n, rowptr, col, val, None, f"banded_{n}_{bw}")
+
+
+BATCH_KINDS = ("banded", "zipf", "supernodal")
+
+
+def batch_systems(count: int = 4096, seed: int = 3, nmin: int = 256, nmax: int = 65536):
+    """Independent systems for the batched solver, as one block-diagonal UpperCSR.
+
+    Draw order from numpy.random.default_rng(seed): per system, n ~ log-uniform[nmin, nmax]
+    (rounded down), kind uniform over banded / random Zipf / random supernodal, then a
+    per-system seed; banded bandwidth ~ U{4..64}; Zipf rows as random_zipf; supernodal with
+    widths U[8,32] and 1-3 ancestors (sized so the system has about n rows).
+    Returns (U, sys_ptr int32[count+1], kinds list).
+    """
+    rng = np.random.default_rng(seed)
+    parts, kinds, sys_ptr = [], [], [0]
+    for _ in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        kind = BATCH_KINDS[int(rng.integers(0, 3))]
+        s = int(rng.integers(0, 2**31 - 1))
+        if kind == "banded":
+            U = banded(n, int(rng.integers(4, 65)), seed=s)
+        elif kind == "zipf":
+            U = random_zipf(n, seed=s)
+        else:
+            U = random_supernodal(max(1, n // 20), seed=s, wmin=8, wmax=32, max_anc=3)
+        parts.append(U)
+        kinds.append(kind)
+        sys_ptr.append(sys_ptr[-1] + U.n)
+    sys_ptr = np.array(sys_ptr, dtype=np.int64)
+    nnz_ptr = np.concatenate([[0], np.cumsum([p.nnz for p in parts])])
+    rowptr = np.concatenate([[0]] + [p.rowptr[1:] + nnz_ptr[i] for i, p in enumerate(parts)]).astype(np.int64)
+    col = np.concatenate([p.col.astype(np.int64) + sys_ptr[i] for i, p in enumerate(parts)]).astype(np.int32)
+    val = np.concatenate([p.val for p in parts])
+    total = UpperCSR(int(sys_ptr[-1]), rowptr, col, val, None, f"batch_{count}_{seed}")
+    return total, sys_ptr.astype(np.int32), kinds
+
+
 def rhs_for(U: UpperCSR, m: int = 1, seed: int = 0, layout: str = "col"):
     """x_true ~ U(-1,1) and b = U x_true.
 

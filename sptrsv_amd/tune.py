@@ -1,0 +1,75 @@
+"""Autotuner and learned kernel selector for the batched solver.
+
+* exhaustive(): one timed launch per configuration (every system on it), device time of every
+  system -> an (nsys x K) table; configurations invalid for m are +inf.
+* Selector: the 24 features of a system -> the configuration, by a 24-64-64-K ReLU network
+  trained on the GPU (sptrsv_mlp_*) on the autotuner's own labels (argmin time).
+* lpt_order(): longest-predicted-time-first queue order for the persistent launch.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+
+K = capi.BCFG_COUNT
+
+
+def valid_mask(nsys: int, m: int) -> np.ndarray:
+    mask = np.ones((nsys, K), dtype=np.float32)
+    if m > 1:
+        mask[:, capi.BCFG_CHAIN] = 0.0
+    return mask
+
+
+def exhaustive(batch: capi.Batch, d_b, d_x, m: int = 1, precision: int = capi.FP64, reps: int = 2) -> np.ndarray:
+    """Device microseconds of every system under every configuration (min over reps)."""
+    table = np.full((batch.nsys, K), np.inf)
+    mask = valid_mask(batch.nsys, m)
+    for c in range(K):
+        if mask[0, c] == 0:
+            continue
+        best = None
+        for _ in range(reps):
+            t = batch.time_config(c, d_b, d_x, m, precision)
+            best = t if best is None else np.minimum(best, t)
+        table[:, c] = best
+    return table
+
+
+def lpt_order(cost: np.ndarray) -> np.ndarray:
+    """Largest predicted cost first (longest-processing-time list scheduling)."""
+    return np.argsort(-np.asarray(cost), kind="stable").astype(np.int32)
+
+
+class Selector:
+    """Feature -> configuration network, trained and evaluated on the GPU."""
+
+    def __init__(self, seed: int = 0, hidden: int = 64):
+        self.mlp = capi.MLP(24, hidden, hidden, K, seed=seed)
+        self.mu = np.zeros(24, dtype=np.float32)
+        self.sd = np.ones(24, dtype=np.float32)
+
+    def _norm(self, feats):
+        return ((np.asarray(feats, dtype=np.float32) - self.mu) / self.sd).astype(np.float32)
+
+    def fit(self, feats: np.ndarray, times: np.ndarray, m: int = 1, epochs: int = 600, lr: float = 5e-3) -> float:
+        feats = np.asarray(feats, dtype=np.float32)
+        self.mu = feats.mean(axis=0)
+        self.sd = feats.std(axis=0) + 1e-6
+        y = np.argmin(times, axis=1).astype(np.int32)
+        return self.mlp.train(self._norm(feats), y, valid_mask(len(y), m), epochs=epochs, lr=lr)
+
+    def predict(self, feats: np.ndarray, m: int = 1) -> np.ndarray:
+        logits = self.mlp.forward(self._norm(feats))
+        logits[valid_mask(len(logits), m) == 0] = -np.inf
+        return np.argmax(logits, axis=1).astype(np.int32)
+
+    def close(self):
+        self.mlp.close()
+
+
+def regret(times: np.ndarray, choice: np.ndarray) -> float:
+    """Sum of chosen device times over the sum of the best ones, minus one."""
+    chosen = times[np.arange(len(choice)), choice]
+    return float(chosen.sum() / times.min(axis=1).sum() - 1.0)

@@ -1,0 +1,185 @@
+"""GPU: batched persistent solves, graph replays, selector features, the MLP kernels and the
+learned selector's gain over the fallback configuration."""
+import numpy as np
+import pytest
+
+from oracle import features as F
+from oracle import mlp_ref
+from oracle import oracle as O
+from sptrsv_amd import capi, gen, tune
+
+pytestmark = pytest.mark.gpu
+FP64_TOL = 1e-12
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    U, sp, kinds = gen.batch_systems(48, seed=3, nmin=64, nmax=3000)
+    banner(f"batch n={U.n} nnz={U.nnz}")
+    B = capi.Batch(U, sp)
+    yield U, sp, kinds, B
+    B.close()
+
+
+def oracle_rowmajor(U, b):
+    if b.ndim == 1:
+        return O.base_gauss_upper(U.rowptr, U.col, U.val, b)[0]
+    x, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, np.ascontiguousarray(b.T))
+    return np.ascontiguousarray(x.T)
+
+
+def solve_batch(B, b, m, precision=capi.FP64, graph=False):
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    B.solve(d_b, d_x, m, precision, graph)
+    B.sync()
+    return d_x.download()
+
+
+@pytest.mark.parametrize("cfg", range(capi.BCFG_COUNT))
+@pytest.mark.parametrize("m", [1, 4, 16])
+def test_batch_every_config(small_batch, cfg, m):
+    U, sp, kinds, B = small_batch
+    if cfg == capi.BCFG_CHAIN and m > 1:
+        pytest.skip("chain configuration is m == 1 (the kernel falls back to LS1)")
+    banner(f"batch cfg={capi.BCFG_NAMES[cfg]} m={m}")
+    B.set_configs(np.full(B.nsys, cfg), np.arange(B.nsys)[::-1])
+    xt, b = gen.rhs_for(U, m, seed=4, layout="row")
+    x = solve_batch(B, b, m)
+    ref = oracle_rowmajor(U, b)
+    exact = (cfg == capi.BCFG_LS1) or m > 1
+    if exact:
+        assert np.array_equal(x, ref), O.rel_error(ref, x)
+    for s in range(B.nsys):  # every system on its own
+        lo, hi = sp[s], sp[s + 1]
+        assert O.rel_error(ref[lo:hi], x[lo:hi]) <= FP64_TOL, (s, kinds[s])
+
+
+def test_batch_mixed_configs_order_and_fp32(small_batch):
+    U, sp, kinds, B = small_batch
+    rng = np.random.default_rng(0)
+    cfg = rng.integers(0, capi.BCFG_COUNT, B.nsys)
+    B.set_configs(cfg, rng.permutation(B.nsys))
+    xt, b = gen.rhs_for(U, 1, seed=5)
+    ref = oracle_rowmajor(U, b)
+    assert O.rel_error(ref, solve_batch(B, b, 1)) <= FP64_TOL
+    assert O.rel_error(ref, solve_batch(B, b, 1, capi.FP32)) <= 2e-5
+    with pytest.raises(capi.SptrsvError):
+        B.set_configs(order=np.zeros(B.nsys, dtype=np.int32))
+
+
+def test_batch_graph_replays(small_batch):
+    """The captured graph is replayed several times with new right-hand sides uploaded by
+    ordinary (eager) copies in between; every replay is checked."""
+    U, sp, kinds, B = small_batch
+    B.set_configs(np.full(B.nsys, capi.BCFG_LS4), np.arange(B.nsys))
+    d_b = capi.DeviceArray((U.n,))
+    d_x = capi.DeviceArray((U.n,))
+    for rep in range(4):
+        xt, b = gen.rhs_for(U, 1, seed=100 + rep)
+        d_b.upload(b)
+        B.solve(d_b, d_x, 1, capi.FP64, True)
+        B.sync()
+        assert O.rel_error(oracle_rowmajor(U, b), d_x.download()) <= FP64_TOL, rep
+
+
+def test_batch_rejects_cross_system_columns():
+    U = gen.random_zipf(100, seed=1)
+    with pytest.raises(capi.SptrsvError) as e:
+        capi.Batch(U, np.array([0, 50, 100], dtype=np.int32))
+    assert e.value.code == capi.ERR_BAD_INPUT
+
+
+def test_features_match_restatement(small_batch):
+    U, sp, kinds, B = small_batch
+    f = B.features(m=4)
+    lv, _ = O.levels(U.rowptr, U.col)
+    for s in range(B.nsys):
+        ref = F.features(U.rowptr, U.col, U.val, lv, sp[s], sp[s + 1], m=4)
+        np.testing.assert_allclose(f[s], ref, rtol=2e-6, atol=1e-6, err_msg=f"system {s} ({kinds[s]})")
+
+
+def test_time_config_and_exhaustive(small_batch):
+    U, sp, kinds, B = small_batch
+    xt, b = gen.rhs_for(U, 4, seed=6, layout="row")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    T = tune.exhaustive(B, d_b, d_x, m=4, reps=1)
+    assert np.all(np.isinf(T[:, capi.BCFG_CHAIN]))
+    assert np.all(np.isfinite(T[:, :capi.BCFG_CHAIN])) and np.all(T[:, :capi.BCFG_CHAIN] > 0)
+
+
+# ---- the selector network ---------------------------------------------------------------------
+
+def test_mlp_gradient_matches_fp64_reference():
+    rng = np.random.default_rng(1)
+    net = capi.MLP(24, 64, 64, 5, seed=3)
+    X = rng.normal(0, 1, (300, 24)).astype(np.float32)
+    y = rng.integers(0, 5, 300).astype(np.int32)
+    mask = np.ones((300, 5), dtype=np.float32)
+    mask[::2, 4] = 0
+    y[::2] = np.where(y[::2] == 4, 0, y[::2])
+    p = net.get_params()
+    losses, g = net.loss_grad(X, y, mask)
+    rl, rg = mlp_ref.loss_grad(p, net.sizes, X, y, mask)
+    np.testing.assert_allclose(losses, rl, rtol=2e-5, atol=2e-6)
+    np.testing.assert_allclose(g, rg, rtol=2e-4, atol=2e-6)
+
+
+def test_mlp_gradient_finite_differences_on_device_loss():
+    rng = np.random.default_rng(2)
+    net = capi.MLP(24, 64, 64, 5, seed=4)
+    X = rng.normal(0, 1, (64, 24)).astype(np.float32)
+    y = rng.integers(0, 5, 64).astype(np.int32)
+    p = net.get_params().astype(np.float64)
+    _, g = net.loss_grad(X, y)
+    eps = 2e-3
+    idx = rng.choice(len(p), 20, replace=False)
+    for k in idx:
+        pp = p.copy(); pp[k] += eps
+        net.set_params(pp)
+        lp = net.loss_grad(X, y)[0].astype(np.float64).mean()
+        pm = p.copy(); pm[k] -= eps
+        net.set_params(pm)
+        lm = net.loss_grad(X, y)[0].astype(np.float64).mean()
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[k]) <= 3e-3 + 3e-2 * abs(g[k]), (k, fd, g[k])
+
+
+def test_mlp_training_fits_a_separable_task():
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (2000, 24)).astype(np.float32)
+    y = (np.argmax(X[:, :5], axis=1)).astype(np.int32)
+    net = capi.MLP(24, 64, 64, 5, seed=5)
+    l0 = net.loss_grad(X, y)[0].mean()
+    final = net.train(X, y, epochs=400, lr=1e-2)
+    pred = np.argmax(net.forward(X), axis=1)
+    assert final < 0.5 * l0 and (pred == y).mean() > 0.9
+
+
+def test_selector_gains_over_fallback():
+    """Train on autotuner labels of one batch (seed 4), apply to another (seed 3): the learned
+    choice must cost less device time than the fixed fallback configuration and approach the
+    exhaustive optimum."""
+    Ut, spt, _ = gen.batch_systems(160, seed=4, nmin=64, nmax=3000)
+    Ue, spe, _ = gen.batch_systems(96, seed=3, nmin=64, nmax=3000)
+    Bt, Be = capi.Batch(Ut, spt), capi.Batch(Ue, spe)
+    res = {}
+    for name, (U, B) in {"train": (Ut, Bt), "eval": (Ue, Be)}.items():
+        xt, b = gen.rhs_for(U, 1, seed=7)
+        d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray((U.n,))
+        res[name] = (B.features(1), tune.exhaustive(B, d_b, d_x, m=1, reps=2))
+    sel = tune.Selector(seed=0)
+    sel.fit(*res["train"], m=1, epochs=600)
+    fe, Te = res["eval"]
+    choice = sel.predict(fe, m=1)
+    fallback = np.full(len(choice), capi.BCFG_LS1)
+    r_sel, r_fb = tune.regret(Te, choice), tune.regret(Te, fallback)
+    banner(f"selector regret {r_sel:.3f}  fallback regret {r_fb:.3f}")
+    assert r_sel < r_fb
+    assert r_sel < 0.5
+    Bt.close(); Be.close(); sel.close()

@@ -28,6 +28,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """One line per finished section on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_probe(nbytes, reps):
+    """Plain device-to-device copies of a known size: the calibration point for the
+    FETCH_SIZE / WRITE_SIZE counters (read nbytes and write nbytes per copy)."""
+    from sptrsv_amd import capi
+    src = capi.DeviceArray.from_host(np.ones(nbytes // 8))
+    dst = capi.DeviceArray((nbytes // 8,))
+    for _ in range(reps):
+        capi.check(capi.lib().sptrsv_memcpy_d2d(dst.ptr, src.ptr, nbytes), "memcpy_d2d")
+    capi.device_sync()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def alg_bytes(n, nnz, m, vbytes=8):
     return nnz * (vbytes + 4) + 4 * (n + 1) + 2 * vbytes * n * m
 
@@ -103,6 +120,103 @@ def cpu_baseline(U, m, budget_s):
     }
 
 
+def bench_mixed(capi, gen, names, out):
+    """Time to a refined solution (max ||d||/||x|| <= 1e-14 -> relative error ~1e-15..1e-13 vs
+    FP64) against the plain FP64 solve of the same kernel family, data resident on the GPU."""
+    from sptrsv_amd import gen as G
+    cases = [("ic0_128", "level_pull", 32, capi.FP32), ("ic0_128", "level_pull", 128, capi.FP32),
+             ("zipf_1e6", "chain", 1, capi.FP32), ("supernodal_4096", "supernodal", 128, capi.BF16),
+             ("supernodal_4096", "supernodal", 32, capi.BF16)]
+    cache = {}
+    for wl, vname, m, prec in cases:
+        key = f"{wl}/{vname}/m{m}"
+        try:
+            if wl not in cache:
+                U, _ = make_workload(wl)
+                P = capi.Plan.from_upper(U)
+                if wl.startswith("supernodal"):
+                    P.build_supernodes(U.snodes)
+                cache = {wl: (U, P)}  # keep one workload resident at a time
+            U, P = cache[wl]
+            _, b = G.rhs_for(U, m, seed=1, layout="row")
+            d_b = capi.DeviceArray.from_host(b)
+            d_x = capi.DeviceArray(b.shape)
+            d_r = capi.DeviceArray(b.shape)
+            prm = capi.params(variant=names[vname])
+            P.solve(d_b, d_x, m, prm)
+            P.sync()
+            t64 = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                P.solve(d_b, d_x, m, prm)
+                P.sync()
+                t64.append(time.perf_counter() - t0)
+            x64 = d_x.download()
+            P.solve_refined(d_b, d_r, m, prm, precision=prec)  # warm-up (builds reduced copies)
+            tref = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                it, ratio = P.solve_refined(d_b, d_r, m, prm, precision=prec)
+                tref.append(time.perf_counter() - t0)
+            xr = d_r.download()
+            err = float(np.linalg.norm(xr - x64) / np.linalg.norm(x64))
+            out[key] = {"fp64_ms": round(min(t64) * 1e3, 3), "refined_ms": round(min(tref) * 1e3, 3),
+                        "speedup": round(min(t64) / min(tref), 3), "iters": it, "last_ratio": ratio,
+                        "rel_err_vs_fp64": err, "low_precision": "bf16+fp32" if prec == capi.BF16 else "fp32"}
+            for d in (d_b, d_x, d_r):
+                d.free()
+        except Exception as exc:
+            out[key] = {"error": str(exc)}
+        progress(f"mixed {key}: {out[key]}")
+
+
+def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, m=1):
+    """Batched solve of `count` systems (n log-uniform [256, nmax], banded / Zipf / supernodal)
+    in one persistent graph-replayed launch: fallback configuration vs exhaustive autotuning
+    vs the MLP selector trained on a separate batch (seed 4)."""
+    from sptrsv_amd import tune
+    Ut, spt, _ = gen.batch_systems(train_count, seed=4, nmin=256, nmax=nmax)
+    Bt = capi.Batch(Ut, spt)
+    _, bt = gen.rhs_for(Ut, m, seed=2, layout="row")
+    dbt, dxt = capi.DeviceArray.from_host(bt), capi.DeviceArray(bt.shape)
+    Tt = tune.exhaustive(Bt, dbt, dxt, m=m, reps=2)
+    sel = tune.Selector(seed=0)
+    loss = sel.fit(Bt.features(m), Tt, m=m, epochs=800)
+    Bt.close()
+    progress(f"batch: selector trained on {train_count} systems, final loss {loss:.4f}")
+    del Ut, dbt, dxt
+    U, sp, kinds = gen.batch_systems(count, seed=3, nmin=256, nmax=nmax)
+    B = capi.Batch(U, sp)
+    _, b = gen.rhs_for(U, m, seed=1, layout="row")
+    d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+    T = tune.exhaustive(B, d_b, d_x, m=m, reps=2)
+    feats = B.features(m)
+    choices = {"fallback_ls1": np.full(B.nsys, capi.BCFG_LS1), "exhaustive": np.argmin(T, axis=1),
+               "mlp": sel.predict(feats, m=m)}
+    res = {"systems": count, "n_total": U.n, "nnz_total": U.nnz, "m": m, "nmax": nmax,
+           "train_systems": train_count, "train_final_loss": loss}
+    for name, ch in choices.items():
+        cost = T[np.arange(B.nsys), ch]
+        B.set_configs(ch, tune.lpt_order(cost))
+        B.solve(d_b, d_x, m, capi.FP64, True)
+        B.sync()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            B.solve(d_b, d_x, m, capi.FP64, True)
+            B.sync()
+            ts.append(time.perf_counter() - t0)
+        best = min(ts)
+        res[name] = {"ms": round(best * 1e3, 3), "gnnz_per_s": round(U.nnz * m / best / 1e9, 3),
+                     "sum_device_us": round(float(cost.sum()), 1), "regret": round(tune.regret(T, ch), 4)}
+        progress(f"batch {name}: {res[name]}")
+    res["mlp_vs_exhaustive_throughput"] = round(res["exhaustive"]["ms"] / res["mlp"]["ms"], 4)
+    res["mlp_accuracy"] = float((choices["mlp"] == choices["exhaustive"]).mean())
+    out["batch"] = res
+    sel.close()
+    B.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,12 +228,18 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run N-byte device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the solve path here is single-GPU "
               "(multi-GPU sharding is design-only for this build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe, 5)
+        return
     from oracle import oracle as O
     from sptrsv_amd import capi, gen
 
@@ -128,6 +248,8 @@ def main():
     U, desc = make_workload(args.workload)
     m = args.m
     P = capi.Plan.from_upper(U)
+    if variant == capi.VAR_SUPERNODAL:
+        P.build_supernodes(U.snodes)
     xt, b = gen.rhs_for(U, m, seed=1, layout="row")
     d_b = capi.DeviceArray.from_host(b)
     d_x = capi.DeviceArray(b.shape)
@@ -142,6 +264,7 @@ def main():
     gbs = B / (ms / 1e3) / 1e9
     achieved = B / (kms / 1e3) / 1e9
     name, cus, mem = capi.device_info()
+    progress(f"{args.workload}/{args.variant}/m{m}: {ms:.4f} ms/step, kernel {kms:.4f} ms, rel err {err:.2e}")
 
     extras = {}
     if not args.no_extras:
@@ -180,8 +303,20 @@ def main():
                 dx.free()
             except Exception as exc:  # report, never hide
                 extras[f"{wl}/{vname}/m{mm}"] = {"error": str(exc)}
+            progress(f"extra {wl}/{vname}/m{mm}: {extras[f'{wl}/{vname}/m{mm}']}")
 
     cpu = None if args.no_cpu else cpu_baseline(U, m, args.cpu_budget)
+    if cpu:
+        progress(f"cpu baseline: {cpu['sample']}")
+    del P
+    mixed, batch = {}, {}
+    if not args.no_mixed:
+        bench_mixed(capi, gen, names, mixed)
+    if not args.no_batch:
+        try:
+            bench_batch(capi, gen, batch)
+        except Exception as exc:
+            batch["error"] = str(exc)
     out = {
         "metric": "SpTRSV GB/s & % HBM peak; time-to-1e-12 error, mixed+refine vs FP64 sync-free",
         "value": round(gbs, 3),
@@ -206,6 +341,8 @@ def main():
                      "kernel_ms": round(kms, 5), "alg_bytes": B},
         "cpu_baseline": cpu,
         "extras": extras,
+        "mixed_refine": mixed,
+        "batched": batch,
     }
     print(json.dumps(out))
 

@@ -68,7 +68,12 @@ typedef struct {
     int unroll;       /* entries polled per step in the pull kernels (1, 2 or 4); 0 = default */
     int precision;    /* SPTRSV_FP64 / SPTRSV_FP32 */
     int watchdog_us;  /* per-wait device watchdog in microseconds; 0 = default (20 s) */
-    int reserved[3];
+    int occupancy;    /* pull kernels: at most this many resident workgroups per CU; < 0 = the
+                         hardware limit; 0 = default: LEVEL_PULL about three mean level widths of
+                         rows in flight, ROW_PULL the hardware limit */
+    int block;        /* pull kernels: threads per workgroup (64..1024, power of two); 0 =
+                         default: LEVEL_PULL 64 rows per workgroup, ROW_PULL 256 / 1024 threads */
+    int reserved[1];
 } sptrsv_params_t;
 
 typedef struct sptrsv_plan_s* sptrsv_plan_t;

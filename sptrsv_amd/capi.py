@@ -52,7 +52,8 @@ class RefineOpts(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("group", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("precision", ctypes.c_int), ("watchdog_us", ctypes.c_int),
-                ("reserved", ctypes.c_int * 3)]
+                ("occupancy", ctypes.c_int), ("block", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 _lib = None
@@ -136,9 +137,10 @@ def check(rc: int, where: str):
         raise SptrsvError(rc, where, lib().sptrsv_last_error().decode())
 
 
-def params(variant=VAR_AUTO, group=0, unroll=0, precision=FP64, watchdog_us=0) -> Params:
+def params(variant=VAR_AUTO, group=0, unroll=0, precision=FP64, watchdog_us=0, occupancy=0, block=0) -> Params:
     p = Params()
     p.variant, p.group, p.unroll, p.precision, p.watchdog_us = variant, group, unroll, precision, watchdog_us
+    p.occupancy, p.block = occupancy, block
     return p
 
 

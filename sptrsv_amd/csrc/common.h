@@ -30,6 +30,7 @@ int fail(int code, const std::string& msg);
 
 int ensure_device();
 uint64_t wallclock_ticks_per_ms();
+int device_compute_units();
 
 // ---- device-side helpers -------------------------------------------------------------------
 constexpr int kWave = 64;

@@ -50,6 +50,35 @@ double gpu_upper_csr(const char* where, int n, const uint64_t* rowptr, const int
     return ms / 1000.0;
 }
 
+// Supernodal solve through the given supernode partition (SUPERNODAL variant); a partition
+// whose rows do not have the supernodal structure is reported and solved by the AUTO variant.
+double gpu_upper_supernodal(const char* where, int n, const uint64_t* rowptr, const int* col, const double* val,
+                            double* x, const double* b, int rhs, size_t sn, const int* snodes) {
+    std::vector<int64_t> rp(rowptr, rowptr + n + 1);
+    sptrsv_plan_t plan = nullptr;
+    int rc = sptrsv_plan_create_csr(n, rp[n], rp.data(), col, val, &plan);
+    if (rc != SPTRSV_OK) {
+        report(where, rc);
+        return rc;
+    }
+    sptrsv_params_t prm{};
+    prm.variant = SPTRSV_VAR_SUPERNODAL;
+    if (sptrsv_plan_build_supernodes(plan, (int)sn, snodes) != SPTRSV_OK) {
+        std::fprintf(stderr, "%s: %s; solving without supernodes\n", where, sptrsv_last_error());
+        prm.variant = SPTRSV_VAR_AUTO;
+    }
+    sptrsv_plan_timing(plan, 1);
+    rc = sptrsv_solve_host(plan, &prm, b, x, rhs, 1);
+    double ms = 0.0;
+    if (rc == SPTRSV_OK) rc = sptrsv_plan_timing_read(plan, &ms, nullptr);
+    sptrsv_plan_destroy(plan);
+    if (rc != SPTRSV_OK) {
+        report(where, rc);
+        return rc;
+    }
+    return ms / 1000.0;
+}
+
 // Supernode rows of the padded format hold d leading zeros (row d of a supernode); drop them
 // to get back the plain upper CSR (the inverse of ccs2ccs_pad, src/core.cpp:136-161).
 void unpad(int n, size_t sn, const int* snodes, const double* val_pad, const uint64_t* col_pad,
@@ -111,10 +140,8 @@ double base_gauss_upper_new(int n, double* val, uint64_t* row_index, int* col, d
 
 double supernodal_upper(size_t sn, int* supernodes, double* x, double* val, int* row, uint64_t* col_index, int n,
                         int rhs) {
-    (void)sn;
-    (void)supernodes;
     std::vector<double> b(x, x + (size_t)n * rhs);
-    return gpu_upper_csr("supernodal_upper", n, col_index, row, val, x, b.data(), rhs, 1, SPTRSV_VAR_AUTO);
+    return gpu_upper_supernodal("supernodal_upper", n, col_index, row, val, x, b.data(), rhs, sn, supernodes);
 }
 
 double supernodal_upper_new(size_t sn, int* supernodes, double* x, double* val, int* row, uint64_t* col_index, int n,
@@ -130,8 +157,8 @@ double supernodal_blas_upper(int n, int nz, size_t sn, int* supernodes, double* 
     std::vector<double> v;
     unpad(n, sn, supernodes, val, col_index, row, rp, c, v);
     std::vector<double> b(x, x + (size_t)n * rhs);
-    return gpu_upper_csr("supernodal_blas_upper", n, rp.data(), c.data(), v.data(), x, b.data(), rhs, 1,
-                         SPTRSV_VAR_AUTO);
+    return gpu_upper_supernodal("supernodal_blas_upper", n, rp.data(), c.data(), v.data(), x, b.data(), rhs, sn,
+                                supernodes);
 }
 
 double supernodal_blas_upper_new(int n, int nz, size_t sn, int* supernodes, double* x, double* val,

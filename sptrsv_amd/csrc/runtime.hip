@@ -17,6 +17,7 @@ static thread_local std::string g_last_error;
 static std::mutex g_dev_mutex;
 static int g_device = -1;
 static uint64_t g_ticks_per_ms = 0;
+static int g_compute_units = 0;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -36,11 +37,15 @@ int ensure_device() {
     int khz = 0;
     SPT_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
     g_ticks_per_ms = khz > 0 ? (uint64_t)khz : 100000;
+    int cus = 0;
+    SPT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+    g_compute_units = cus > 0 ? cus : 256;
     g_device = 0;
     return SPTRSV_OK;
 }
 
 uint64_t wallclock_ticks_per_ms() { return g_ticks_per_ms ? g_ticks_per_ms : 100000; }
+int device_compute_units() { return g_compute_units ? g_compute_units : 256; }
 
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out) {
     *stop_out = nullptr;

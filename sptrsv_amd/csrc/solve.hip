@@ -50,6 +50,10 @@ struct PullArgs {
 constexpr int kPullBlock = 1024;
 constexpr int kPullBlockRow = 256;
 
+// The row's entries are taken in windows of UNROLL.  A window's indices and values are loaded
+// once; each pass polls only the window's entries still holding the sentinel (an x value, once
+// seen, never changes), then adds the ready prefix in ascending order.  Re-polling only the
+// missing values keeps the wait traffic to about one load per waiting lane and pass.
 template <typename T, int G, int KPL, int UNROLL, bool LEVELED, int BS>
 __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
 #pragma clang fp contract(off)
@@ -59,17 +63,32 @@ __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
     const int slot = t * (BS / G) + (int)threadIdx.x / G;
     const int gl = lane % G;
     bool done = slot >= a.n || gl >= a.kc;
-    int row_csr = 0, row_out = 0, p = 0, end = 0;
+    int row_out = 0, p = 0, end = 0, cnt = 0, q = 0;
     T diag = T(1);
-    T acc[KPL];
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) acc[j] = T(0);
+    T acc[KPL], bv[KPL];
+    int c[UNROLL];
+    T v[UNROLL];
+    U xb[UNROLL][KPL];
     const size_t m = (size_t)a.m;
-    T bv[KPL];
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) bv[j] = T(0);
+    for (int j = 0; j < KPL; ++j) acc[j] = bv[j] = T(0);
+    auto load_window = [&]() {
+        cnt = min(UNROLL, end - p);
+        q = 0;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            c[u] = 0;
+            v[u] = T(0);
+            if (u < cnt) {
+                c[u] = a.col[p + u];
+                v[u] = a.val[p + u];
+            }
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) xb[u][j] = Bits<T>::sentinel;
+        }
+    };
     if (!done) {
-        row_csr = LEVELED ? slot : a.n - 1 - slot;
+        const int row_csr = LEVELED ? slot : a.n - 1 - slot;
         row_out = LEVELED ? a.perm[slot] : row_csr;
         const int s = a.rowptr[row_csr];
         diag = a.val[s];
@@ -78,6 +97,7 @@ __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
 #pragma unroll
         for (int j = 0; j < KPL; ++j)
             if (gl + j * G < a.kc) bv[j] = a.b[(size_t)row_out * m + a.k0 + gl + j * G];
+        load_window();
     }
     wait_vmem();
     Watchdog wd;
@@ -86,57 +106,45 @@ __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
     while (true) {
         bool progressed = false;
         if (!done) {
-            while (p < end) {
-                const int cnt = min(UNROLL, end - p);
-                int c[UNROLL];
-                T v[UNROLL];
-                U xb[UNROLL][KPL];
 #pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    if (u < cnt) {
-                        c[u] = a.col[p + u];
-                        v[u] = a.val[p + u];
-                    }
+            for (int u = 0; u < UNROLL; ++u) {
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) {
+                    const int k = gl + j * G;
+                    if (u >= q && u < cnt && k < a.kc && xb[u][j] == Bits<T>::sentinel)
+                        xb[u][j] = poll_bits(a.x + (size_t)c[u] * m + a.k0 + k);
                 }
+            }
+            bool stop = false;
 #pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-#pragma unroll
-                    for (int j = 0; j < KPL; ++j) {
-                        const int k = gl + j * G;
-                        xb[u][j] = Bits<T>::sentinel;
-                        if (u < cnt && k < a.kc) xb[u][j] = poll_bits(a.x + (size_t)c[u] * m + a.k0 + k);
-                    }
-                }
-                int ok = 0;
-                bool stop = false;
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    bool ready = u < cnt && !stop;
+            for (int u = 0; u < UNROLL; ++u) {
+                if (u >= q && u < cnt && !stop) {
+                    bool ready = true;
 #pragma unroll
                     for (int j = 0; j < KPL; ++j)
                         if (gl + j * G < a.kc && xb[u][j] == Bits<T>::sentinel) ready = false;
                     if (ready) {
 #pragma unroll
                         for (int j = 0; j < KPL; ++j) acc[j] = acc[j] + v[u] * from_bits<T>(xb[u][j]);
-                        ++ok;
+                        q = u + 1;
+                        progressed = true;
                     } else {
                         stop = true;
                     }
                 }
-                p += ok;
-                if (ok) progressed = true;
-                if (ok < cnt) break;
             }
-            if (p >= end) {
+            if (q == cnt) {
+                p += cnt;
+                if (p < end) {
+                    load_window();
+                } else {
 #pragma unroll
-                for (int j = 0; j < KPL; ++j) {
-                    const int k = gl + j * G;
-                    if (k < a.kc) {
-                        const size_t at = (size_t)row_out * m + a.k0 + k;
-                        publish(a.x + at, (bv[j] - acc[j]) / diag);
+                    for (int j = 0; j < KPL; ++j) {
+                        const int k = gl + j * G;
+                        if (k < a.kc) publish(a.x + (size_t)row_out * m + a.k0 + k, (bv[j] - acc[j]) / diag);
                     }
+                    done = true;
                 }
-                done = true;
                 progressed = true;
             }
         }
@@ -384,14 +392,52 @@ inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
     return (int)g;
 }
 
+// Dynamic LDS that lets at most `occ` workgroups of a pull kernel reside on one CU (160 KiB of
+// LDS per CU; occ <= 0 = no cap).  Fewer resident rows far ahead of the dependency front means
+// fewer lanes polling x values that cannot be ready yet: every poll of an unpublished value is
+// a device-scope load that crosses the fabric, and thousands of them in flight lengthen the
+// latency of the loads that matter.
+inline size_t occupancy_lds(int occ) {
+    if (occ <= 0) return 0;
+    return (size_t)(163840 / occ - 512) & ~(size_t)255;
+}
+
+// Default pull workgroup: 64 rows (one wave at one lane per row), at most 1024 threads.
+inline int default_block(int G) { return std::min(1024, 64 * G); }
+
+// Default residency: about three mean level widths of rows in flight over the whole chip
+// (measured on the 128^3 IC(0) factor, whose mean level holds 5,490 rows: one 64-row workgroup
+// per CU beats the hardware limit 2.9x at one right-hand side).  Wide-level factors get the
+// hardware limit back, since their rows do not wait.
+inline int default_occupancy(const sptrsv_plan_s* P, int rows_per_block, int bs) {
+    const double width = (double)P->n / std::max(P->n_levels, 1);
+    const double rows_per_unit = (double)device_compute_units() * rows_per_block;
+    const int occ = std::max(1, (int)std::lround(3.0 * width / rows_per_unit));
+    const int hw = std::max(1, 2048 / bs);  // 32 waves per CU
+    return occ >= hw ? -1 : occ;
+}
+
 inline int next_pow2(int v) {
     int g = 1;
     while (g < v) g <<= 1;
     return g;
 }
 
+// One pull launch with workgroup size `bs` (64..1024 threads, BS / G rows per workgroup).
+template <typename T, int G, int K, int U, bool LEVELED>
+void run_pull(int bs, int n, size_t lds, hipStream_t s, const PullArgs<T>& a) {
+    const dim3 grid((unsigned)(((int64_t)n * G + bs - 1) / bs));
+    switch (bs) {
+        case 64: k_pull<T, G, K, U, LEVELED, 64><<<grid, 64, lds, s>>>(a); break;
+        case 128: k_pull<T, G, K, U, LEVELED, 128><<<grid, 128, lds, s>>>(a); break;
+        case 256: k_pull<T, G, K, U, LEVELED, 256><<<grid, 256, lds, s>>>(a); break;
+        case 512: k_pull<T, G, K, U, LEVELED, 512><<<grid, 512, lds, s>>>(a); break;
+        default: k_pull<T, G, K, U, LEVELED, 1024><<<grid, 1024, lds, s>>>(a); break;
+    }
+}
+
 template <typename T, bool LEVELED>
-int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int unroll) {
+int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int unroll, int occ, int block) {
     // lanes per row: m == 1 -> 1 (or `group` lanes splitting entries); m <= 64 -> next pow2 of
     // m; m > 64 -> 64 lanes with up to 4 columns each, launched in chunks of 256 columns.
     hipStream_t s = P->stream;
@@ -424,28 +470,27 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
         a.kc = kc;
         const int G = kc == 1 ? 1 : std::min(64, next_pow2(kc));
         const int kpl = (kc + G - 1) / G;  // 1..4
-        const int bs = G == 1 ? kPullBlockRow : kPullBlock;
-        const int rows_per_block = bs / G;
-        const dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block));
+        // Row order needs the dependency distance in flight, not a few levels: its default is
+        // the hardware limit with 256- or 1024-thread workgroups.
+        int bs = block > 0 ? next_pow2(std::min(std::max(block, 64), 1024))
+                           : (LEVELED ? default_block(G) : (G == 1 ? kPullBlockRow : kPullBlock));
+        bs = std::max(bs, G);
+        const size_t lds = occupancy_lds(occ != 0 ? occ : LEVELED ? default_occupancy(P, bs / G, bs) : -1);
         hipEvent_t stop;
         SPT_CHECK(begin_timed(P, &stop));
         k_reset_ticket<<<1, 64, 0, s>>>(a.ctrl);
-#define SPT_PULL(G_, K_, U_) k_pull<T, G_, K_, U_, LEVELED, kPullBlock><<<grid, kPullBlock, 0, s>>>(a)
-#define SPT_PULL1(U_) k_pull<T, 1, 1, U_, LEVELED, kPullBlockRow><<<grid, kPullBlockRow, 0, s>>>(a)
         if (G == 1) {
-            if (unroll == 1) SPT_PULL1(1);
-            else if (unroll == 2) SPT_PULL1(2);
-            else SPT_PULL1(4);
-        } else if (G == 2) SPT_PULL(2, 1, 2);
-        else if (G == 4) SPT_PULL(4, 1, 2);
-        else if (G == 8) SPT_PULL(8, 1, 2);
-        else if (G == 16) SPT_PULL(16, 1, 2);
-        else if (G == 32) SPT_PULL(32, 1, 2);
-        else if (kpl == 1) SPT_PULL(64, 1, 2);
-        else if (kpl == 2) SPT_PULL(64, 2, 1);
-        else SPT_PULL(64, 4, 1);
-#undef SPT_PULL
-#undef SPT_PULL1
+            if (unroll == 1) run_pull<T, 1, 1, 1, LEVELED>(bs, n, lds, s, a);
+            else if (unroll == 2) run_pull<T, 1, 1, 2, LEVELED>(bs, n, lds, s, a);
+            else run_pull<T, 1, 1, 4, LEVELED>(bs, n, lds, s, a);
+        } else if (G == 2) run_pull<T, 2, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (G == 4) run_pull<T, 4, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (G == 8) run_pull<T, 8, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (G == 16) run_pull<T, 16, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (G == 32) run_pull<T, 32, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (kpl == 1) run_pull<T, 64, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (kpl == 2) run_pull<T, 64, 2, 1, LEVELED>(bs, n, lds, s, a);
+        else run_pull<T, 64, 4, 1, LEVELED>(bs, n, lds, s, a);
         end_timed(P, stop);
         SPT_HIP(hipGetLastError());
     }
@@ -453,7 +498,7 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
 }
 
 template <typename T>
-int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, const T* d_b, T* d_x, int m,
+int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, int occ, int block, const T* d_b, T* d_x, int m,
                 const T* val, const T* cval, const T* lval, uint64_t limit) {
     hipStream_t s = P->stream;
     const int n = P->n;
@@ -474,13 +519,13 @@ int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, const T* d
                 a.col = P->col;
                 a.val = val;
                 a.perm = nullptr;
-                return launch_pull<T, false>(P, a, m, group, unroll);
+                return launch_pull<T, false>(P, a, m, group, unroll, occ, block);
             }
             a.rowptr = P->lrowptr;
             a.col = P->lcol;
             a.val = lval;
             a.perm = P->level_rows;
-            return launch_pull<T, true>(P, a, m, group, unroll);
+            return launch_pull<T, true>(P, a, m, group, unroll, occ, block);
         }
         case SPTRSV_VAR_CSC_PUSH: {
             const size_t need = (size_t)nm * sizeof(T);
@@ -543,9 +588,11 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
     return SPTRSV_OK;
 }
 
+// AUTO: one right-hand side over a factor whose levels hold fewer than four rows on average
+// (a dependency chain) goes to the single-wave chain kernel, which keeps each window of rows in
+// LDS instead of paying a device-wide round trip per level; everything else to LEVEL_PULL.
 int default_variant(const sptrsv_plan_s* P, int m) {
-    (void)m;
-    (void)P;
+    if (m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
     return SPTRSV_VAR_LEVEL_PULL;
 }
 
@@ -570,7 +617,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_CHAIN) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
-        return solve_typed<double>(P, variant, prm.group, unroll, d_b, d_x, m, P->val, P->cval, P->lval, limit);
+        return solve_typed<double>(P, variant, prm.group, unroll, prm.occupancy, prm.block, d_b, d_x, m, P->val, P->cval, P->lval, limit);
     if (prm.precision == SPTRSV_FP32) {
         SPT_CHECK(ensure_fp32(P));
         const int64_t nm = (int64_t)P->n * m;
@@ -583,7 +630,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
             P->stage_elems = (size_t)nm;
         }
         k_convert<double, float><<<grid_for(nm), 256, 0, P->stream>>>(d_b, P->b32, nm);
-        SPT_CHECK(solve_typed<float>(P, variant, prm.group, unroll, P->b32, P->x32, m, P->val32, P->cval32,
+        SPT_CHECK(solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, P->b32, P->x32, m, P->val32, P->cval32,
                                      P->lval32, limit));
         k_convert<float, double><<<grid_for(nm), 256, 0, P->stream>>>(P->x32, d_x, nm);
         SPT_HIP(hipGetLastError());
@@ -674,7 +721,7 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     auto solve_low = [&](const float* in, float* out) -> int {
         if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve_low(P, low, in, out, m, limit);
         if (variant == SPTRSV_VAR_CHAIN) return chain_solve32(P, in, out);
-        return solve_typed<float>(P, variant, prm.group, unroll, in, out, m, P->val32, P->cval32, P->lval32, limit);
+        return solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, in, out, m, P->val32, P->cval32, P->lval32, limit);
     };
     const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));
     std::vector<unsigned long long> h((size_t)2 * m);

@@ -174,6 +174,37 @@ def test_fp64_multi_rhs(plans, m, variant):
             assert O.rel_error(ref, x) <= FP64_TOL
 
 
+@pytest.mark.parametrize("block", [64, 128, 1024])
+@pytest.mark.parametrize("occupancy", [-1, 1, 2])
+def test_pull_launch_shapes(plans, block, occupancy):
+    """Workgroup size and residency cap change only the schedule: the level-ordered and
+    row-ordered pull kernels stay bitwise equal to the serial oracle."""
+    for name in ("zipf_3000", "ic0_12", "chain_4000"):
+        U, P = MATS[name], plans[name]
+        for m in (1, 8, 128):
+            banner(f"shape {name} m={m} block={block} occ={occupancy}")
+            xt, b = gen.rhs_for(U, m, seed=21, layout="row")
+            ref = oracle_solve(U, b)
+            for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL):
+                x = P.solve_host(b, capi.params(variant=variant, block=block, occupancy=occupancy))
+                assert np.array_equal(x, ref), (name, m, variant, O.rel_error(ref, x))
+
+
+def test_auto_variant(plans):
+    """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
+    m == 1 and everything else to the level-ordered pull kernel (bitwise exact)."""
+    for name in MATS:
+        U, P = MATS[name], plans[name]
+        for m in (1, 3):
+            xt, b = gen.rhs_for(U, m, seed=22, layout="row")
+            ref = oracle_solve(U, b)
+            x = P.solve_host(b, capi.params())
+            if m == 1 and P.n_levels * 4 > U.n:
+                assert O.rel_error(ref, x) <= FP64_TOL, name
+            else:
+                assert np.array_equal(x, ref), (name, m)
+
+
 def test_column_major_host_layout(plans):
     U, P = MATS["zipf_3000"], plans["zipf_3000"]
     xt, b = gen.rhs_for(U, 4, seed=6, layout="col")  # (m, n)

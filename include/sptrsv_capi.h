@@ -151,14 +151,17 @@ int sptrsv_residual(sptrsv_plan_t plan, const double* d_b, const double* d_x, do
 
 /* Mixed precision with FP64 iterative refinement (no reference counterpart).
  * x_0 = solve_low(b); then repeat  r = b - U x (FP64 residual on the GPU),  d = solve_low(r),
- * x += d (FP64)  until  max_k ||d_k||_inf / ||x_k||_inf <= tol  or max_iters corrections.
- * solve_low is the kernel chosen by params->variant in opts->precision (FP32; BF16 only with
- * the supernodal variant).  Synchronous; *iters = corrections applied, *ratio = last ratio.
- * Reaching max_iters is not an error: check *ratio against tol. */
+ * x += d (FP64).  With r_k = max over columns of ||d_k||_inf / ||x_k||_inf (r_0 = 1), the error
+ * left after correction k is estimated as r_k * min(1, r_k / r_{k-1}); the loop stops when that
+ * estimate is <= tol, when a correction shrinks by less than half (k >= 2: the FP64 residual
+ * floor), or after max_iters corrections.  solve_low is the kernel chosen by params->variant in
+ * opts->precision (FP32; BF16 only with the supernodal variant).  Synchronous; *iters =
+ * corrections applied, *ratio = the error estimate of the returned x (converged when <= tol).
+ * Reaching max_iters or the floor without meeting tol is not an error: check *ratio. */
 typedef struct {
     int max_iters;    /* 0 -> 30 */
     int precision;    /* SPTRSV_FP32 or SPTRSV_BF16 */
-    double tol;       /* 0 -> 1e-14 */
+    double tol;       /* 0 -> 1e-13 (estimated relative error of x) */
 } sptrsv_refine_t;
 int sptrsv_solve_refined(sptrsv_plan_t plan, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
                          const double* d_b, double* d_x, int m, int* iters, double* ratio);

@@ -344,7 +344,7 @@ class Plan:
 
     def solve_refined(self, d_b, d_x, m=1, prm=None, precision=FP32, tol=0.0, max_iters=0):
         """Reduced-precision solves + FP64 refinement on the device (synchronous).
-        Returns (corrections applied, last max ||d||/||x||)."""
+        Returns (corrections applied, estimated relative error of x; converged when <= tol)."""
         o = RefineOpts()
         o.max_iters, o.precision, o.tol = max_iters, precision, tol
         it, ratio = _i(0), ctypes.c_double(0.0)

@@ -689,7 +689,7 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     sptrsv_refine_t ro{};
     if (opts) ro = *opts;
     const int max_iters = ro.max_iters > 0 ? ro.max_iters : 30;
-    const double tol = ro.tol > 0 ? ro.tol : 1e-14;
+    const double tol = ro.tol > 0 ? ro.tol : 1e-13;
     const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
     int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
     if (variant == SPTRSV_VAR_TILE)
@@ -731,7 +731,10 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 1);
     SPT_HIP(hipGetLastError());
     SPT_CHECK(sptrsv_plan_sync(P));
-    double last = 1.0;
+    // Stopping rule: correction k has relative size r_k ~ the error of x before it; with the
+    // observed contraction c = r_k / r_{k-1} the error left after it is about r_k * c.  Stop when
+    // that estimate is <= tol, or when the corrections stop shrinking (the FP64 residual floor).
+    double last = 1.0, prev = 1.0, estimate = 1.0;
     int done = 0;
     for (int it = 1; it <= max_iters; ++it) {
         k_residual32<<<grid_for(nm), 256, 0, s>>>(n, m, P->rowptr, P->col, P->val, d_x, d_b, P->ref_r32);
@@ -748,10 +751,13 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
             last = std::max(last, xn > 0 ? dn / xn : dn);
         }
         done = it;
-        if (!(last > tol)) break;
+        estimate = last * std::min(1.0, last / prev);
+        if (!(estimate > tol)) break;
+        if (it >= 2 && last > 0.5 * prev) break;
+        prev = last;
     }
     if (iters) *iters = done;
-    if (ratio) *ratio = last;
+    if (ratio) *ratio = estimate;
     return SPTRSV_OK;
 }
 

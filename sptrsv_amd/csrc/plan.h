@@ -67,7 +67,8 @@ struct sptrsv_plan_s {
     float* sn_rscale = nullptr;     // per-supernode scale of the BF16 block
     int* sn_dep_ptr = nullptr;   // sn + 1
     int* sn_dep = nullptr;       // supernodes owning the rectangular columns
-    int* sn_flag = nullptr;      // sn, epoch of the last solve that published the supernode
+    int* sn_flag = nullptr;      // sn x sn_flag_blocks epochs of the last solve that published
+    int sn_flag_blocks = 1;      //   (supernode, column block); the reduced-precision kernel uses [k]
     int sn_epoch = 0;
     std::vector<int> h_snodes;
     std::vector<int> h_sn_r;        // rectangular column count per supernode

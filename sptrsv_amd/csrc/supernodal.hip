@@ -12,10 +12,11 @@
 //     X_s = Tinv_s * W                   (w x m, K = w)      v_mfma_f64_16x16x4f64
 // v_mfma_f64_16x16x4f64 lane layout (measured on gfx950): A(i,k) in lane i + 16k, B(k,j) in
 // lane j + 16k, result element v of lane l is D(l/16 + 4v, l%16).
-// One 256-thread workgroup per supernode, handed out by tickets from the last supernode
-// down; a workgroup waits (acquire) for the epoch flags of the supernodes owning its
-// rectangular columns -- all later in ticket order, so already running -- and publishes its
-// own flag (release) after writing X_s.  W lives in LDS (w <= 128 rows x MC columns).
+// FP64: one wave per (supernode, 16-column block of X), handed out by tickets from the last
+// supernode down; the wave waits (acquire) for the same column block of the supernodes owning
+// its rectangular columns -- earlier in ticket order, so already running -- and publishes its
+// own (supernode, block) flag (release) after writing X_s.  W stays in the accumulators.
+// Reduced precision (k_sn_solve_low): one 256-thread workgroup per supernode, W in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,6 +53,7 @@ __global__ void k_sn_invert(const int* __restrict__ start, const int64_t* __rest
 struct SnArgs {
     int sn;
     int m;
+    int nblk;          // 16-column blocks of X: ceil(m / 16)
     const int* start;
     const int* rcount;
     const int64_t* offt;
@@ -62,7 +64,7 @@ struct SnArgs {
     const double* rect;
     const int* dep_ptr;
     const int* dep;
-    int* flag;
+    int* flag;         // sn x nblk: epoch of the last solve that published (supernode, block)
     int epoch;
     const double* b;
     double* x;
@@ -70,96 +72,129 @@ struct SnArgs {
     uint64_t limit;
 };
 
-template <int MC>
-__global__ __launch_bounds__(256) void k_sn_solve(SnArgs a) {
-    __shared__ double W[kMaxW * MC];
-    __shared__ int s_ticket;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
-    __syncthreads();
-    const int k = a.sn - 1 - s_ticket;
+constexpr int kSnCChunk = 256;  // rectangular column indices staged in LDS at a time
+
+// One wave per (supernode k, 16-column block J of X), tickets from the last supernode down.
+// The wave waits for block J of every supernode owning one of k's rectangular columns, then
+//   W   = B_s(:, J) - R_s * X(rect, J)     row tiles I of 16: acc[I] (K loop in chunks of 16)
+//   X_s = Tinv_s * W                       in place, tile by tile (X[I] needs W rows >= 16 I)
+// The f64 16x16x4 result layout (lane (q, c) holds rows q + 4v of column c) is exactly its
+// B-operand layout for the K-step of rows 16 I2 + 4 v (lane (q, c) holds row 4 v + q), so
+// Tinv * W reads W straight from the accumulators: element v of acc[I2] in the same lane.
+__global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
+    __shared__ int sC[kSnCChunk];
+    const int lane = threadIdx.x;
+    const int t = wave_ticket(&a.ctrl[0]);
+    const int k = a.sn - 1 - t / a.nblk;
+    const int J = t % a.nblk;
     if (k < 0) return;
     const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
-    if (tid == 0) {
+    {
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
+        const int d1 = a.dep_ptr[k + 1];
         int idle = 0;
         bool give_up = false;
-        for (int q = a.dep_ptr[k]; q < a.dep_ptr[k + 1] && !give_up; ++q) {
-            const int d = a.dep[q];
-            while (__hip_atomic_load(&a.flag[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+        for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
+            const int q = base + lane;
+            bool ready = q >= d1;
+            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.nblk + J];
+            while (!__all(ready)) {
+                if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+                if (__all(ready)) break;
                 __builtin_amdgcn_s_sleep(2);
-                if ((++idle & 63) == 0 && wd.expired()) {
+                if ((++idle & 63) == 0 && __any(wd.expired())) {
                     give_up = true;
                     break;
                 }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    const int nI = (w + 15) >> 4;
-    constexpr int nJ = MC / 16;
-    const int tiles = nI * nJ;
     const int c16 = lane & 15, quad = lane >> 4;
-    const double* R = a.rect + a.offr[k];
-    const double* Ti = a.tinv + a.offt[k];
-    const int* C = a.cols + a.offc[k];
     const size_t m = (size_t)a.m;
-    for (int j0 = 0; j0 < a.m; j0 += MC) {
-        // W = B_s - R_s * X(rect, j0:j0+MC)
-        for (int t = wave; t < tiles; t += 4) {
-            const int I = t / nJ, J = t % nJ;
-            const int jj = j0 + 16 * J + c16;
-            const bool jv = jj < a.m;
-            v4d acc;
+    const int jj = 16 * J + c16;
+    const bool jv = jj < a.m;
+    const int nI = (w + 15) >> 4;
+    v4d acc[8];
+#pragma unroll
+    for (int I = 0; I < 8; ++I) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + quad + 4 * v;
+            acc[I][v] = (I < nI && i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
+        }
+    }
+    const double* R = a.rect + a.offr[k];
+    const int* C = a.cols + a.offc[k];
+    for (int cb = 0; cb < r; cb += kSnCChunk) {
+        const int cn = min(kSnCChunk, r - cb);
+        __syncthreads();
+        for (int q = lane; q < cn; q += kWave) sC[q] = C[cb + q];
+        __syncthreads();
+        for (int k0 = 0; k0 < cn; k0 += 16) {
+            double bv[4], av[8][4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int kk = k0 + 4 * s + quad;
+                bv[s] = (kk < cn && jv) ? a.x[(size_t)sC[kk] * m + jj] : 0.0;
+            }
+#pragma unroll
+            for (int I = 0; I < 8; ++I) {
+                if (I < nI) {
+                    const int ia = 16 * I + c16;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int kk = k0 + 4 * s + quad;
+                        av[I][s] = (ia < w && kk < cn) ? -R[(size_t)ia * r + cb + kk] : 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+#pragma unroll
+                for (int I = 0; I < 8; ++I)
+                    if (I < nI) acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I][s], bv[s], acc[I], 0, 0, 0);
+            }
+        }
+    }
+
+    const double* Ti = a.tinv + a.offt[k];
+#pragma unroll
+    for (int I = 0; I < 8; ++I) {
+        if (I < nI) {
+            v4d xt = {0.0, 0.0, 0.0, 0.0};
+            const int ia = 16 * I + c16;
+            const bool iav = ia < w;
+#pragma unroll
+            for (int I2 = I; I2 < 8; ++I2) {
+                if (I2 < nI) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int kk = 16 * I2 + 4 * v;
+                        if (kk < w) {
+                            const double av = (iav && kk + quad < w) ? Ti[(size_t)ia * w + kk + quad] : 0.0;
+                            xt = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[I2][v], xt, 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            acc[I] = xt;
+        }
+    }
+#pragma unroll
+    for (int I = 0; I < 8; ++I) {
+        if (I < nI) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = 16 * I + quad + 4 * v;
-                acc[v] = (i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
-            }
-            const int ia = 16 * I + c16;
-            const bool iav = ia < w;
-            const double* Rrow = R + (size_t)(iav ? ia : 0) * r;
-#pragma unroll 4
-            for (int k0 = 0; k0 < r; k0 += 4) {
-                const int kk = k0 + quad;
-                const bool kv = kk < r;
-                const double av = (iav && kv) ? -Rrow[kk] : 0.0;
-                const double bv = (kv && jv) ? a.x[(size_t)C[kv ? kk : 0] * m + jj] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) W[(16 * I + quad + 4 * v) * MC + 16 * J + c16] = acc[v];
-        }
-        __syncthreads();
-        // X_s = Tinv_s * W   (Tinv is upper triangular: row tile I only needs k >= 16 I)
-        for (int t = wave; t < tiles; t += 4) {
-            const int I = t / nJ, J = t % nJ;
-            const int jj = j0 + 16 * J + c16;
-            v4d acc = {0.0, 0.0, 0.0, 0.0};
-            const int ia = 16 * I + c16;
-            const bool iav = ia < w;
-            const double* Trow = Ti + (size_t)(iav ? ia : 0) * w;
-#pragma unroll 4
-            for (int k0 = 16 * I; k0 < w; k0 += 4) {
-                const int kk = k0 + quad;
-                const double av = (iav && kk < w) ? Trow[kk] : 0.0;
-                const double bv = W[kk * MC + 16 * J + c16];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 16 * I + quad + 4 * v;
-                if (i < w && jj < a.m) a.x[(size_t)(s0 + i) * m + jj] = acc[v];
+                if (i < w && jv) a.x[(size_t)(s0 + i) * m + jj] = acc[I][v];
             }
         }
-        __syncthreads();
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) __hip_atomic_store(&a.flag[(size_t)k * a.nblk + J], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_reset_ticket_sn(int* ctrl) {
@@ -341,6 +376,20 @@ __global__ __launch_bounds__(256) void k_sn_solve_low(SnLowArgs a) {
     if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Ready flags for sn x nblk (supernode, column block) pairs; a new array starts at 0, below
+// every epoch in use.
+int ensure_sn_flags(sptrsv_plan_s* P, int nblk) {
+    if (nblk <= P->sn_flag_blocks) return SPTRSV_OK;
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    if (P->sn_flag) SPT_HIP(hipFree(P->sn_flag));
+    P->sn_flag = nullptr;
+    const size_t count = (size_t)P->sn * nblk;
+    SPT_HIP(hipMalloc((void**)&P->sn_flag, count * sizeof(int)));
+    SPT_HIP(hipMemset(P->sn_flag, 0, count * sizeof(int)));
+    P->sn_flag_blocks = nblk;
+    return SPTRSV_OK;
+}
+
 int ensure_sn_low(sptrsv_plan_s* P) {
     if (P->sn_rect16) return SPTRSV_OK;
     const int S = P->sn;
@@ -372,6 +421,7 @@ void supernodal_free(sptrsv_plan_s* p) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     p->sn_start = p->sn_r = p->sn_offc = p->sn_cols = p->sn_dep_ptr = p->sn_dep = p->sn_flag = nullptr;
+    p->sn_flag_blocks = 1;
     p->sn_offt = p->sn_offr = nullptr;
     p->sn_tinv = p->sn_rect = nullptr;
     p->sn_tinv32 = nullptr;
@@ -389,15 +439,15 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
     if (precision != SPTRSV_FP64)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant: FP64 only through sptrsv_solve");
+    const int nblk = (m + 15) / 16;
+    SPT_CHECK(ensure_sn_flags(P, nblk));
     P->sn_epoch++;
-    SnArgs a{P->sn, m, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
+    SnArgs a{P->sn, m, nblk, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
              P->sn_rect, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch, d_b, d_x, P->ticket, limit};
     k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (m <= 16) k_sn_solve<16><<<P->sn, 256, 0, P->stream>>>(a);
-    else if (m <= 32) k_sn_solve<32><<<P->sn, 256, 0, P->stream>>>(a);
-    else k_sn_solve<64><<<P->sn, 256, 0, P->stream>>>(a);
+    k_sn_solve<<<(unsigned)((int64_t)P->sn * nblk), kWave, 0, P->stream>>>(a);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
@@ -542,6 +592,7 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
     P->h_sn_r = rcount;
     P->sn_wmax = wmax;
     P->sn_epoch = 0;
+    P->sn_flag_blocks = 1;
     P->h_snodes = sb;
     return SPTRSV_OK;
 }

@@ -472,7 +472,13 @@ def test_bf16_supernodal_refined_reaches_fp64(m):
         xt, b = gen.rhs_for(U, m, seed=11, layout="row")
         x, it, est = _refined(P, b, m, capi.VAR_SUPERNODAL, capi.BF16)
         assert est <= 1e-13 and it <= 12, (it, est)
-        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+        ref = oracle_solve(U, b)
+        assert O.rel_error(ref, x) <= FP64_TOL
+        # the FP64 kernel after the reduced-precision one on the same plan (shared epoch flags)
+        for mm in (m, 2 * m + 3):
+            xt2, b2 = gen.rhs_for(U, mm, seed=12, layout="row")
+            x2 = P.solve_host(b2, capi.params(variant=capi.VAR_SUPERNODAL))
+            assert O.rel_error(oracle_solve(U, b2), x2) <= FP64_TOL, mm
 
 
 def test_refinement_reports_non_convergence():

@@ -61,6 +61,7 @@ struct sptrsv_plan_s {
     int* sn_offc = nullptr;      // sn, offset into sn_cols
     int* sn_cols = nullptr;
     double* sn_tinv = nullptr;
+    double* sn_tri = nullptr;     // diagonal blocks (row-major w x w, zero below), for the residual
     double* sn_rect = nullptr;
     float* sn_tinv32 = nullptr;     // mixed precision: FP32 inverse blocks
     uint16_t* sn_rect16 = nullptr;  // mixed precision: BF16 rectangular blocks, scaled per block
@@ -68,7 +69,7 @@ struct sptrsv_plan_s {
     int* sn_dep_ptr = nullptr;   // sn + 1
     int* sn_dep = nullptr;       // supernodes owning the rectangular columns
     int* sn_flag = nullptr;      // sn x sn_flag_blocks epochs of the last solve that published
-    int sn_flag_blocks = 1;      //   (supernode, column block); the reduced-precision kernel uses [k]
+    int sn_flag_blocks = 1;      //   (supernode, column panel); the reduced-precision kernel uses [k]
     int sn_epoch = 0;
     std::vector<int> h_snodes;
     std::vector<int> h_sn_r;        // rectangular column count per supernode
@@ -107,6 +108,7 @@ int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, cons
 int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
                      uint64_t limit);  // supernodal.hip
 void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip
+int supernodal_residual32(sptrsv_plan_s* p, const double* d_b, const double* d_x, float* d_r32, int m);
 int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
                          uint64_t limit);  // supernodal.hip
 int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,

@@ -737,7 +737,10 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     double last = 1.0, prev = 1.0, estimate = 1.0;
     int done = 0;
     for (int it = 1; it <= max_iters; ++it) {
-        k_residual32<<<grid_for(nm), 256, 0, s>>>(n, m, P->rowptr, P->col, P->val, d_x, d_b, P->ref_r32);
+        if (variant == SPTRSV_VAR_SUPERNODAL)
+            SPT_CHECK(supernodal_residual32(P, d_b, d_x, P->ref_r32, m));
+        else
+            k_residual32<<<grid_for(nm), 256, 0, s>>>(n, m, P->rowptr, P->col, P->val, d_x, d_b, P->ref_r32);
         SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
         k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
         k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 0);

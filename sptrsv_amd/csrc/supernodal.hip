@@ -12,10 +12,10 @@
 //     X_s = Tinv_s * W                   (w x m, K = w)      v_mfma_f64_16x16x4f64
 // v_mfma_f64_16x16x4f64 lane layout (measured on gfx950): A(i,k) in lane i + 16k, B(k,j) in
 // lane j + 16k, result element v of lane l is D(l/16 + 4v, l%16).
-// FP64: one wave per (supernode, 16-column block of X), handed out by tickets from the last
-// supernode down; the wave waits (acquire) for the same column block of the supernodes owning
+// FP64: one workgroup per (supernode, panel of up to 64 columns of X), handed out by tickets
+// from the last supernode down; it waits (acquire) for the same panel of the supernodes owning
 // its rectangular columns -- earlier in ticket order, so already running -- and publishes its
-// own (supernode, block) flag (release) after writing X_s.  W stays in the accumulators.
+// own (supernode, panel) flag (release) after writing X_s.  W stays in the accumulators.
 // Reduced precision (k_sn_solve_low): one 256-thread workgroup per supernode, W in LDS.
 #include <hip/hip_runtime.h>
 
@@ -53,7 +53,7 @@ __global__ void k_sn_invert(const int* __restrict__ start, const int64_t* __rest
 struct SnArgs {
     int sn;
     int m;
-    int nblk;          // 16-column blocks of X: ceil(m / 16)
+    int npanel;        // column panels per supernode: ceil(ceil(m / 16) / NW)
     const int* start;
     const int* rcount;
     const int64_t* offt;
@@ -62,34 +62,58 @@ struct SnArgs {
     const int* cols;
     const double* tinv;
     const double* rect;
+    const double* tri;  // diagonal blocks T_s (residual mode)
     const int* dep_ptr;
     const int* dep;
-    int* flag;         // sn x nblk: epoch of the last solve that published (supernode, block)
+    int* flag;         // sn x npanel: epoch of the last solve that published (supernode, panel)
     int epoch;
     const double* b;
     double* x;
+    float* r32;        // residual mode: r = b - U x, rounded to FP32
     int* ctrl;
     uint64_t limit;
 };
 
-constexpr int kSnCChunk = 256;  // rectangular column indices staged in LDS at a time
+constexpr int kSnCChunk = 1024;    // rectangular column indices staged in LDS at a time
+constexpr int kSnLdt = kMaxW + 1;  // LDS row stride (doubles) of a 16-row block of Tinv
 
-// One wave per (supernode k, 16-column block J of X), tickets from the last supernode down.
-// The wave waits for block J of every supernode owning one of k's rectangular columns, then
-//   W   = B_s(:, J) - R_s * X(rect, J)     row tiles I of 16: acc[I] (K loop in chunks of 16)
-//   X_s = Tinv_s * W                       in place, tile by tile (X[I] needs W rows >= 16 I)
+// One workgroup of NW waves per (supernode k, panel of NW 16-column blocks of X), tickets from
+// the last supernode down.  Wave 0 waits for the same panel of the supernodes owning k's
+// rectangular columns, then every wave J of the panel computes, for its 16 columns,
+//   W   = B_s - R_s * X(rect)    row tiles I of 16 in acc[I]; K in chunks of KC: the chunk of
+//                                R goes through a double-buffered LDS tile shared by the waves,
+//                                the gathered X rows straight into B-operand registers, and
+//                                the next chunk is loaded while the current one is multiplied
+//   X_s = Tinv_s * W             in place, row block by row block (X[I] needs W rows >= 16 I);
+//                                Tinv row blocks staged through the same LDS.
 // The f64 16x16x4 result layout (lane (q, c) holds rows q + 4v of column c) is exactly its
 // B-operand layout for the K-step of rows 16 I2 + 4 v (lane (q, c) holds row 4 v + q), so
 // Tinv * W reads W straight from the accumulators: element v of acc[I2] in the same lane.
-__global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
+// RESID: the FP64 residual of the refinement, r_s = B_s - T_s X_s - R_s X(rect) (no waits; the
+// triangle is one more K segment with columns s0 .. s0 + w - 1), stored as FP32.
+template <int NW, bool RESID>
+__global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve(SnArgs a) {
+    constexpr int NT = 64 * NW;
+    constexpr int kSnKC = NW == 4 ? 32 : 16;  // K chunk (register budget: 2 waves per SIMD)
+    constexpr int kSnLdr = kSnKC + 1;         // LDS row stride (doubles) of a K chunk of R
+    constexpr int kSnBuf = 2 * kMaxW * kSnLdr;
+    static_assert(kSnBuf >= 2 * 16 * kSnLdt, "Tinv blocks must fit the R double buffer");
+    constexpr int kPerR = (kMaxW * kSnKC + NT - 1) / NT;  // R chunk elements loaded per thread
+    constexpr int kPerT = (16 * kMaxW + NT - 1) / NT;  // Tinv block elements loaded per thread
+    __shared__ double sbuf[kSnBuf];
     __shared__ int sC[kSnCChunk];
-    const int lane = threadIdx.x;
-    const int t = wave_ticket(&a.ctrl[0]);
-    const int k = a.sn - 1 - t / a.nblk;
-    const int J = t % a.nblk;
+    __shared__ int s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (!RESID) {
+        if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
+        __syncthreads();
+    }
+    const int t = RESID ? (int)blockIdx.x : s_ticket;
+    const int k = a.sn - 1 - t / a.npanel;
+    const int panel = t % a.npanel;
     if (k < 0) return;
     const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
-    {
+    if (!RESID && wv == 0) {
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
         const int d1 = a.dep_ptr[k + 1];
@@ -98,7 +122,7 @@ __global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
         for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
             const int q = base + lane;
             bool ready = q >= d1;
-            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.nblk + J];
+            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.npanel + panel];
             while (!__all(ready)) {
                 if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
                 if (__all(ready)) break;
@@ -109,12 +133,13 @@ __global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
     const int c16 = lane & 15, quad = lane >> 4;
     const size_t m = (size_t)a.m;
-    const int jj = 16 * J + c16;
+    const int jj = 16 * (panel * NW + wv) + c16;
     const bool jv = jj < a.m;
     const int nI = (w + 15) >> 4;
     v4d acc[8];
@@ -126,47 +151,115 @@ __global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
             acc[I][v] = (I < nI && i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
         }
     }
-    const double* R = a.rect + a.offr[k];
-    const int* C = a.cols + a.offc[k];
-    for (int cb = 0; cb < r; cb += kSnCChunk) {
-        const int cn = min(kSnCChunk, r - cb);
-        __syncthreads();
-        for (int q = lane; q < cn; q += kWave) sC[q] = C[cb + q];
-        __syncthreads();
-        for (int k0 = 0; k0 < cn; k0 += 16) {
-            double bv[4], av[8][4];
+
+    // ---- W = B_s - A X(cols) over K segments (solve: R_s; residual: T_s, then R_s) --------
+    double rreg[kPerR];
+    constexpr int KS = kSnKC / 4;  // MFMA K-steps per chunk
+    double bv[KS], bn[KS];
+    // one K segment: A is w x K row-major (leading dimension lda); column kk of the segment is
+    // X row sC[kk] (cols != nullptr, staged in LDS by blocks) or row s0 + kk (cols == nullptr)
+    auto segment = [&](const double* A, int lda, int K, const int* cols) {
+        auto load_r = [&](int kg) {  // A[:, kg .. kg+KC) into registers (zeros outside)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int kk = k0 + 4 * s + quad;
-                bv[s] = (kk < cn && jv) ? a.x[(size_t)sC[kk] * m + jj] : 0.0;
+            for (int q = 0; q < kPerR; ++q) {
+                const int e = tid + q * NT, row = e / kSnKC, kc = kg + e % kSnKC;
+                rreg[q] = (row < w && kc < K) ? A[(size_t)row * lda + kc] : 0.0;
             }
+        };
+        auto store_r = [&](double* dst) {
 #pragma unroll
-            for (int I = 0; I < 8; ++I) {
-                if (I < nI) {
-                    const int ia = 16 * I + c16;
+            for (int q = 0; q < kPerR; ++q) {
+                const int e = tid + q * NT;
+                if (e / kSnKC < kMaxW) dst[(e / kSnKC) * kSnLdr + e % kSnKC] = rreg[q];
+            }
+        };
+        for (int cb = 0; cb < K; cb += kSnCChunk) {
+            const int cn = min(kSnCChunk, K - cb);
+            const int nch = (cn + kSnKC - 1) / kSnKC;
+            auto load_x = [&](double* out, int kl) {  // negated X rows of local chunk kl
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const int kk = k0 + 4 * s + quad;
-                        av[I][s] = (ia < w && kk < cn) ? -R[(size_t)ia * r + cb + kk] : 0.0;
+                for (int s = 0; s < KS; ++s) {
+                    const int kk = kl + 4 * s + quad;
+                    const int row = cols ? sC[kk < cn ? kk : 0] : s0 + cb + kk;
+                    out[s] = (kk < cn && jv) ? -a.x[(size_t)row * m + jj] : 0.0;
+                }
+            };
+            __syncthreads();
+            if (cols)
+                for (int q = tid; q < cn; q += NT) sC[q] = cols[cb + q];
+            __syncthreads();
+            load_x(bn, 0);
+            load_r(cb);
+            store_r(sbuf);
+            __syncthreads();
+            for (int c = 0; c < nch; ++c) {
+                const double* cur = sbuf + (c & 1) * (kMaxW * kSnLdr);
+                double* nxt = sbuf + ((c + 1) & 1) * (kMaxW * kSnLdr);
+#pragma unroll
+                for (int s = 0; s < KS; ++s) bv[s] = bn[s];
+                if (c + 1 < nch) {
+                    load_x(bn, kSnKC * (c + 1));
+                    load_r(cb + kSnKC * (c + 1));
+                }
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+#pragma unroll
+                    for (int I = 0; I < 8; ++I) {
+                        if (I < nI) {
+                            const double av = cur[(16 * I + c16) * kSnLdr + 4 * s + quad];
+                            acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[s], acc[I], 0, 0, 0);
+                        }
                     }
                 }
-            }
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-#pragma unroll
-                for (int I = 0; I < 8; ++I)
-                    if (I < nI) acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[I][s], bv[s], acc[I], 0, 0, 0);
+                if (c + 1 < nch) store_r(nxt);
+                __syncthreads();
             }
         }
+    };
+    if (RESID) segment(a.tri + a.offt[k], w, w, nullptr);
+    segment(a.rect + a.offr[k], r, r, a.cols + a.offc[k]);
+    if (RESID) {
+#pragma unroll
+        for (int I = 0; I < 8; ++I) {
+            if (I < nI) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * I + quad + 4 * v;
+                    if (i < w && jv) a.r32[(size_t)(s0 + i) * m + jj] = (float)acc[I][v];
+                }
+            }
+        }
+        return;
     }
 
+    // ---- X_s = Tinv_s W, row block by row block ------------------------------------------
     const double* Ti = a.tinv + a.offt[k];
+    double treg[kPerT];
+    auto load_t = [&](int I) {  // Tinv rows 16 I .. 16 I + 15, all columns
+#pragma unroll
+        for (int q = 0; q < kPerT; ++q) {
+            const int e = tid + q * NT, row = 16 * I + e / kMaxW, col = e % kMaxW;
+            treg[q] = (e < 16 * kMaxW && row < w && col < w) ? Ti[(size_t)row * w + col] : 0.0;
+        }
+    };
+    auto store_t = [&](double* dst) {
+#pragma unroll
+        for (int q = 0; q < kPerT; ++q) {
+            const int e = tid + q * NT;
+            if (e < 16 * kMaxW) dst[(e / kMaxW) * kSnLdt + e % kMaxW] = treg[q];
+        }
+    };
+    __syncthreads();
+    load_t(0);
+    store_t(sbuf);
+    __syncthreads();
 #pragma unroll
     for (int I = 0; I < 8; ++I) {
         if (I < nI) {
+            const double* cur = sbuf + (I & 1) * (16 * kSnLdt);
+            double* nxt = sbuf + ((I + 1) & 1) * (16 * kSnLdt);
+            if (I + 1 < nI) load_t(I + 1);
             v4d xt = {0.0, 0.0, 0.0, 0.0};
-            const int ia = 16 * I + c16;
-            const bool iav = ia < w;
 #pragma unroll
             for (int I2 = I; I2 < 8; ++I2) {
                 if (I2 < nI) {
@@ -174,13 +267,15 @@ __global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
                     for (int v = 0; v < 4; ++v) {
                         const int kk = 16 * I2 + 4 * v;
                         if (kk < w) {
-                            const double av = (iav && kk + quad < w) ? Ti[(size_t)ia * w + kk + quad] : 0.0;
+                            const double av = cur[c16 * kSnLdt + kk + quad];
                             xt = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[I2][v], xt, 0, 0, 0);
                         }
                     }
                 }
             }
             acc[I] = xt;
+            if (I + 1 < nI) store_t(nxt);
+            __syncthreads();
         }
     }
 #pragma unroll
@@ -194,7 +289,9 @@ __global__ __launch_bounds__(64) void k_sn_solve(SnArgs a) {
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(&a.flag[(size_t)k * a.nblk + J], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (tid == 0)
+        __hip_atomic_store(&a.flag[(size_t)k * a.npanel + panel], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_reset_ticket_sn(int* ctrl) {
@@ -376,7 +473,7 @@ __global__ __launch_bounds__(256) void k_sn_solve_low(SnLowArgs a) {
     if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Ready flags for sn x nblk (supernode, column block) pairs; a new array starts at 0, below
+// Ready flags for sn x npanel (supernode, column panel) pairs; a new array starts at 0, below
 // every epoch in use.
 int ensure_sn_flags(sptrsv_plan_s* P, int nblk) {
     if (nblk <= P->sn_flag_blocks) return SPTRSV_OK;
@@ -417,13 +514,14 @@ int ensure_sn_low(sptrsv_plan_s* P) {
 
 void supernodal_free(sptrsv_plan_s* p) {
     void* bufs[] = {p->sn_start, p->sn_r, p->sn_offt, p->sn_offr, p->sn_offc, p->sn_cols, p->sn_tinv,
-                    p->sn_rect, p->sn_tinv32, p->sn_rect16, p->sn_rscale, p->sn_dep_ptr, p->sn_dep, p->sn_flag};
+                    p->sn_rect, p->sn_tinv32, p->sn_rect16, p->sn_rscale, p->sn_dep_ptr, p->sn_dep, p->sn_flag,
+                    p->sn_tri};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     p->sn_start = p->sn_r = p->sn_offc = p->sn_cols = p->sn_dep_ptr = p->sn_dep = p->sn_flag = nullptr;
     p->sn_flag_blocks = 1;
     p->sn_offt = p->sn_offr = nullptr;
-    p->sn_tinv = p->sn_rect = nullptr;
+    p->sn_tinv = p->sn_rect = p->sn_tri = nullptr;
     p->sn_tinv32 = nullptr;
     p->sn_rect16 = nullptr;
     p->sn_rscale = nullptr;
@@ -440,15 +538,39 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     if (precision != SPTRSV_FP64)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant: FP64 only through sptrsv_solve");
     const int nblk = (m + 15) / 16;
-    SPT_CHECK(ensure_sn_flags(P, nblk));
+    const int nw = nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1;
+    const int npanel = (nblk + nw - 1) / nw;
+    SPT_CHECK(ensure_sn_flags(P, npanel));
     P->sn_epoch++;
-    SnArgs a{P->sn, m, nblk, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
-             P->sn_rect, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch, d_b, d_x, P->ticket, limit};
+    SnArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
+             P->sn_rect, P->sn_tri, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch, d_b, d_x, nullptr,
+             P->ticket, limit};
     k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    k_sn_solve<<<(unsigned)((int64_t)P->sn * nblk), kWave, 0, P->stream>>>(a);
+    const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
+    if (nw == 4) k_sn_solve<4, false><<<grid, 256, 0, P->stream>>>(a);
+    else if (nw == 2) k_sn_solve<2, false><<<grid, 128, 0, P->stream>>>(a);
+    else k_sn_solve<1, false><<<grid, 64, 0, P->stream>>>(a);
     end_timed(P, stop);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+// r = b - U x in FP64 on MFMA through the supernodal blocks, stored as FP32 (refinement).
+int supernodal_residual32(sptrsv_plan_s* P, const double* d_b, const double* d_x, float* d_r32, int m) {
+    if (P->sn == 0)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
+    const int nblk = (m + 15) / 16;
+    const int nw = nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1;
+    const int npanel = (nblk + nw - 1) / nw;
+    SnArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
+             P->sn_rect, P->sn_tri, P->sn_dep_ptr, P->sn_dep, P->sn_flag, 0, d_b, const_cast<double*>(d_x), d_r32,
+             P->ticket, 0};
+    const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
+    if (nw == 4) k_sn_solve<4, true><<<grid, 256, 0, P->stream>>>(a);
+    else if (nw == 2) k_sn_solve<2, true><<<grid, 128, 0, P->stream>>>(a);
+    else k_sn_solve<1, true><<<grid, 64, 0, P->stream>>>(a);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }
@@ -586,7 +708,7 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
     k_sn_invert<<<S, 128, 0, P->stream>>>(P->sn_start, P->sn_offt, d_T, P->sn_tinv);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
-    (void)hipFree(d_T);
+    P->sn_tri = d_T;  // kept for the FP64 residual
     if (e != hipSuccess) return fail(SPTRSV_ERR_HIP, std::string("supernode inversion: ") + hipGetErrorString(e));
     P->sn = S;
     P->h_sn_r = rcount;

@@ -69,7 +69,7 @@ struct sptrsv_plan_s {
     int* sn_dep_ptr = nullptr;   // sn + 1
     int* sn_dep = nullptr;       // supernodes owning the rectangular columns
     int* sn_flag = nullptr;      // sn x sn_flag_blocks epochs of the last solve that published
-    int sn_flag_blocks = 1;      //   (supernode, column panel); the reduced-precision kernel uses [k]
+    int sn_flag_blocks = 1;      //   (supernode, column panel)
     int sn_epoch = 0;
     std::vector<int> h_snodes;
     std::vector<int> h_sn_r;        // rectangular column count per supernode

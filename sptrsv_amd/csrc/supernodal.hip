@@ -16,7 +16,7 @@
 // from the last supernode down; it waits (acquire) for the same panel of the supernodes owning
 // its rectangular columns -- earlier in ticket order, so already running -- and publishes its
 // own (supernode, panel) flag (release) after writing X_s.  W stays in the accumulators.
-// Reduced precision (k_sn_solve_low): one 256-thread workgroup per supernode, W in LDS.
+// Reduced precision (k_sn_solve_low): the same decomposition with BF16 / FP32 MFMA.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -355,16 +355,14 @@ __global__ void k_sn_to_float(const double* __restrict__ in, float* __restrict__
 struct SnLowArgs {
     int sn;
     int m;
-    int bf16;  // rectangular product in BF16 (else FP32 on the VALU)
+    int npanel;
     const int* start;
     const int* rcount;
     const int64_t* offt;
-    const int64_t* offr;
     const int64_t* offr16;
     const int* offc;
     const int* cols;
     const float* tinv;
-    const double* rect;
     const uint16_t* rect16;
     const float* rscale;
     const int* dep_ptr;
@@ -377,26 +375,48 @@ struct SnLowArgs {
     uint64_t limit;
 };
 
-template <int MC>
-__global__ __launch_bounds__(256) void k_sn_solve_low(SnLowArgs a) {
-    __shared__ float W[kMaxW * MC];
+// Reduced-precision solve, same decomposition and waits as k_sn_solve:
+//   W   = B_s - scale_s * (R16_s * bf16(X(rect)))   v_mfma_f32_16x16x16bf16_1k, FP32 accumulate;
+//                                                   K chunks of 32 of R16 double-buffered in LDS
+//   X_s = Tinv_s * W                                v_mfma_f32_16x16x4f32; W of each wave through
+//                                                   LDS (the f32 result layout is not its B layout)
+template <int NW>
+__global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve_low(SnLowArgs a) {
+    constexpr int NT = 64 * NW;
+    constexpr int KC = 32;                         // bf16 K chunk: 2 MFMA steps of 16
+    constexpr int LDR = KC + 8;                    // LDS row stride in bf16 (80 B, 8-byte aligned)
+    constexpr int G = KC / 4;                      // 4-element groups per chunk row
+    constexpr int kPerR = (kMaxW * G + NT - 1) / NT;
+    constexpr int KS = KC / 16;
+    constexpr int LDT = kMaxW + 1;                 // LDS row stride (floats) of a Tinv row block
+    static_assert(2 * kMaxW * LDR * 2 >= 2 * 16 * LDT * 4, "Tinv blocks must fit the R buffer");
+    __shared__ __align__(16) uint16_t sR[2 * kMaxW * LDR];
+    __shared__ float sW[NW][kMaxW][17];
+    __shared__ int sC[kSnCChunk];
     __shared__ int s_ticket;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
     __syncthreads();
-    const int k = a.sn - 1 - s_ticket;
+    const int t = s_ticket;
+    const int k = a.sn - 1 - t / a.npanel;
+    const int panel = t % a.npanel;
     if (k < 0) return;
     const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
-    if (tid == 0) {
+    if (wv == 0) {
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
+        const int d1 = a.dep_ptr[k + 1];
         int idle = 0;
         bool give_up = false;
-        for (int q = a.dep_ptr[k]; q < a.dep_ptr[k + 1] && !give_up; ++q) {
-            const int d = a.dep[q];
-            while (__hip_atomic_load(&a.flag[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+        for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
+            const int q = base + lane;
+            bool ready = q >= d1;
+            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.npanel + panel];
+            while (!__all(ready)) {
+                if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+                if (__all(ready)) break;
                 __builtin_amdgcn_s_sleep(2);
-                if ((++idle & 63) == 0 && wd.expired()) {
+                if ((++idle & 63) == 0 && __any(wd.expired())) {
                     give_up = true;
                     break;
                 }
@@ -406,71 +426,134 @@ __global__ __launch_bounds__(256) void k_sn_solve_low(SnLowArgs a) {
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    const int nI = (w + 15) >> 4;
-    constexpr int nJ = MC / 16;
-    const int tiles = nI * nJ;
     const int c16 = lane & 15, quad = lane >> 4;
+    const size_t m = (size_t)a.m;
+    const int jj = 16 * (panel * NW + wv) + c16;
+    const bool jv = jj < a.m;
+    const int nI = (w + 15) >> 4;
     const int rp = (r + 15) & ~15;
     const uint16_t* R16 = a.rect16 + a.offr16[k];
-    const float* Ti = a.tinv + a.offt[k];
     const int* C = a.cols + a.offc[k];
-    const float sc = a.rscale[k];
-    const size_t m = (size_t)a.m;
-    for (int j0 = 0; j0 < a.m; j0 += MC) {
-        // W = B_s - scale * (R16 * bf16(X(rect, :)))
-        for (int t = wave; t < tiles; t += 4) {
-            const int I = t / nJ, J = t % nJ;
-            const int jj = j0 + 16 * J + c16;
-            const bool jv = jj < a.m;
-            const int ia = 16 * I + c16;
-            const bool iav = ia < w;
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
-            const v4s zero = {0, 0, 0, 0};
-            for (int k0 = 0; k0 < rp; k0 += 16) {
-                const int kb = k0 + 4 * quad;
-                const v4s av = iav ? *reinterpret_cast<const v4s*>(R16 + (size_t)ia * rp + kb) : zero;
-                v4s bvv;
+    v4f acc[8];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int kq = kb + q;
-                    const float xv = (kq < r && jv) ? a.x[(size_t)C[kq] * m + jj] : 0.f;
-                    bvv[q] = to_bf16(xv);
+    for (int I = 0; I < 8; ++I) acc[I] = v4f{0.f, 0.f, 0.f, 0.f};
+    const v4s zero = {0, 0, 0, 0};
+    v4s rreg[kPerR];
+    float bv[KS][4], bn[KS][4];
+    auto load_r = [&](int kg) {  // R16[:, kg .. kg+KC) in 4-element groups (zeros outside)
+#pragma unroll
+        for (int q = 0; q < kPerR; ++q) {
+            const int g = tid + q * NT, row = g / G, kq = kg + 4 * (g % G);
+            rreg[q] = (row < w && kq < rp) ? *reinterpret_cast<const v4s*>(R16 + (size_t)row * rp + kq) : zero;
+        }
+    };
+    auto store_r = [&](uint16_t* dst) {
+#pragma unroll
+        for (int q = 0; q < kPerR; ++q) {
+            const int g = tid + q * NT;
+            if (g / G < kMaxW) *reinterpret_cast<v4s*>(dst + (g / G) * LDR + 4 * (g % G)) = rreg[q];
+        }
+    };
+    for (int cb = 0; cb < r; cb += kSnCChunk) {
+        const int cn = min(kSnCChunk, r - cb);
+        const int nch = (cn + KC - 1) / KC;
+        auto load_x = [&](int kl) {  // gathered X rows of local chunk kl into bn
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int kk = kl + 16 * s + 4 * quad + e;
+                    bn[s][e] = (kk < cn && jv) ? a.x[(size_t)sC[kk < cn ? kk : 0] * m + jj] : 0.f;
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bvv, acc, 0, 0, 0);
             }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 16 * I + 4 * quad + v;
-                W[i * MC + 16 * J + c16] = (i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] - sc * acc[v] : 0.f;
-            }
-        }
+        };
         __syncthreads();
-        // X_s = Tinv_s * W in FP32 on MFMA
-        for (int t = wave; t < tiles; t += 4) {
-            const int I = t / nJ, J = t % nJ;
-            const int jj = j0 + 16 * J + c16;
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
-            const int ia = 16 * I + c16;
-            const bool iav = ia < w;
-            const float* Trow = Ti + (size_t)(iav ? ia : 0) * w;
-#pragma unroll 4
-            for (int k0 = 16 * I; k0 < w; k0 += 4) {
-                const int kk = k0 + quad;
-                const float av = (iav && kk < w) ? Trow[kk] : 0.f;
-                const float bvv = W[kk * MC + 16 * J + c16];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bvv, acc, 0, 0, 0);
+        for (int q = tid; q < cn; q += NT) sC[q] = C[cb + q];
+        __syncthreads();
+        load_x(0);
+        load_r(cb);
+        store_r(sR);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            const uint16_t* cur = sR + (c & 1) * (kMaxW * LDR);
+            uint16_t* nxt = sR + ((c + 1) & 1) * (kMaxW * LDR);
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bv[s][e] = bn[s][e];
+            if (c + 1 < nch) {
+                load_x(KC * (c + 1));
+                load_r(cb + KC * (c + 1));
             }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 16 * I + 4 * quad + v;
-                if (i < w && jj < a.m) a.x[(size_t)(s0 + i) * m + jj] = acc[v];
+            for (int s = 0; s < KS; ++s) {
+                v4s bb;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bb[e] = to_bf16(bv[s][e]);
+#pragma unroll
+                for (int I = 0; I < 8; ++I) {
+                    if (I < nI) {
+                        const v4s av = *reinterpret_cast<const v4s*>(cur + (16 * I + c16) * LDR + 16 * s + 4 * quad);
+                        acc[I] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bb, acc[I], 0, 0, 0);
+                    }
+                }
             }
+            if (c + 1 < nch) store_r(nxt);
+            __syncthreads();
         }
+    }
+    // W = B_s - scale * acc, this wave's 16 columns, into LDS (rows 4 quad + v of each tile)
+    const float sc = a.rscale[k];
+#pragma unroll
+    for (int I = 0; I < 8; ++I) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * quad + v;
+            if (i < kMaxW) sW[wv][i][c16] = (I < nI && i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] - sc * acc[I][v] : 0.f;
+        }
+    }
+    // X_s = Tinv_s W, Tinv row blocks double-buffered in LDS
+    float* sT = reinterpret_cast<float*>(sR);
+    const float* Ti = a.tinv + a.offt[k];
+    constexpr int kPerT = (16 * kMaxW + NT - 1) / NT;
+    float treg[kPerT];
+    auto load_t = [&](int I) {
+#pragma unroll
+        for (int q = 0; q < kPerT; ++q) {
+            const int e = tid + q * NT, row = 16 * I + e / kMaxW, col = e % kMaxW;
+            treg[q] = (e < 16 * kMaxW && row < w && col < w) ? Ti[(size_t)row * w + col] : 0.f;
+        }
+    };
+    auto store_t = [&](float* dst) {
+#pragma unroll
+        for (int q = 0; q < kPerT; ++q) {
+            const int e = tid + q * NT;
+            if (e < 16 * kMaxW) dst[(e / kMaxW) * LDT + e % kMaxW] = treg[q];
+        }
+    };
+    __syncthreads();
+    load_t(0);
+    store_t(sT);
+    __syncthreads();
+    for (int I = 0; I < nI; ++I) {
+        const float* cur = sT + (I & 1) * (16 * LDT);
+        float* nxt = sT + ((I + 1) & 1) * (16 * LDT);
+        if (I + 1 < nI) load_t(I + 1);
+        v4f xt = {0.f, 0.f, 0.f, 0.f};
+        for (int kk = 16 * I; kk < w; kk += 4)
+            xt = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[c16 * LDT + kk + quad], sW[wv][kk + quad][c16], xt, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + 4 * quad + v;
+            if (i < w && jv) a.x[(size_t)(s0 + i) * m + jj] = xt[v];
+        }
+        if (I + 1 < nI) store_t(nxt);
         __syncthreads();
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0)
+        __hip_atomic_store(&a.flag[(size_t)k * a.npanel + panel], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Ready flags for sn x npanel (supernode, column panel) pairs; a new array starts at 0, below
@@ -581,16 +664,21 @@ int supernodal_solve_low(sptrsv_plan_s* P, int precision, const float* d_b32, fl
     if (precision != SPTRSV_BF16)
         return fail(SPTRSV_ERR_UNSUPPORTED, "reduced-precision supernodal solve is BF16 (rectangular) + FP32");
     SPT_CHECK(ensure_sn_low(P));
+    const int nblk = (m + 15) / 16;
+    const int nw = nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1;
+    const int npanel = (nblk + nw - 1) / nw;
+    SPT_CHECK(ensure_sn_flags(P, npanel));
     P->sn_epoch++;
-    SnLowArgs a{P->sn, m, precision == SPTRSV_BF16, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offr16,
-                P->sn_offc, P->sn_cols, P->sn_tinv32, P->sn_rect, P->sn_rect16, P->sn_rscale, P->sn_dep_ptr,
-                P->sn_dep, P->sn_flag, P->sn_epoch, d_b32, d_x32, P->ticket, limit};
+    SnLowArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr16, P->sn_offc, P->sn_cols,
+                P->sn_tinv32, P->sn_rect16, P->sn_rscale, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch,
+                d_b32, d_x32, P->ticket, limit};
     k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (m <= 16) k_sn_solve_low<16><<<P->sn, 256, 0, P->stream>>>(a);
-    else if (m <= 32) k_sn_solve_low<32><<<P->sn, 256, 0, P->stream>>>(a);
-    else k_sn_solve_low<64><<<P->sn, 256, 0, P->stream>>>(a);
+    const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
+    if (nw == 4) k_sn_solve_low<4><<<grid, 256, 0, P->stream>>>(a);
+    else if (nw == 2) k_sn_solve_low<2><<<grid, 128, 0, P->stream>>>(a);
+    else k_sn_solve_low<1><<<grid, 64, 0, P->stream>>>(a);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

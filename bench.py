@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
 
 
 def progress(msg):
@@ -47,6 +48,15 @@ def copy_probe(nbytes, reps):
 
 def alg_bytes(n, nnz, m, vbytes=8):
     return nnz * (vbytes + 4) + 4 * (n + 1) + 2 * vbytes * n * m
+
+
+def supernodal_flops(U, m):
+    """Algorithmic flops of a supernodal solve: per supernode of width w with r rectangular
+    columns, 2*w*r*m (rectangular product) + w*w*m (triangle)."""
+    s = np.asarray(U.snodes, dtype=np.int64)
+    w = np.diff(s)
+    r = (U.rowptr[s[:-1] + 1] - U.rowptr[s[:-1]]) - w
+    return float(np.sum((2 * w * r + w * w) * m))
 
 
 def make_workload(name):
@@ -171,48 +181,66 @@ def bench_mixed(capi, gen, names, out):
         progress(f"mixed {key}: {out[key]}")
 
 
-def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, m=1):
-    """Batched solve of `count` systems (n log-uniform [256, nmax], banded / Zipf / supernodal)
-    in one persistent graph-replayed launch: fallback configuration vs exhaustive autotuning
-    vs the MLP selector trained on a separate batch (seed 4)."""
+def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 4, 16)):
+    """Batched solve of `count` systems (n log-uniform [256, nmax]; banded / Zipf / supernodal)
+    in one persistent graph-replayed launch, for each m: the fallback configuration, exhaustive
+    autotuning, and the MLP selector (one network over all m, trained on the autotuner's labels
+    for a separate batch, seed 4); plus the FP32 + FP64-refinement batched solve."""
     from sptrsv_amd import tune
     Ut, spt, _ = gen.batch_systems(train_count, seed=4, nmin=256, nmax=nmax)
     Bt = capi.Batch(Ut, spt)
-    _, bt = gen.rhs_for(Ut, m, seed=2, layout="row")
-    dbt, dxt = capi.DeviceArray.from_host(bt), capi.DeviceArray(bt.shape)
-    Tt = tune.exhaustive(Bt, dbt, dxt, m=m, reps=2)
-    sel = tune.Selector(seed=0)
-    loss = sel.fit(Bt.features(m), Tt, m=m, epochs=800)
+    feats_t, times_t, masks_t = [], [], []
+    for m in ms:
+        _, bt = gen.rhs_for(Ut, m, seed=2, layout="row")
+        dbt, dxt = capi.DeviceArray.from_host(bt), capi.DeviceArray(bt.shape)
+        T = tune.exhaustive(Bt, dbt, dxt, m=m, reps=2)
+        feats_t.append(Bt.features(m))
+        times_t.append(T)
+        dbt.free(); dxt.free()
     Bt.close()
-    progress(f"batch: selector trained on {train_count} systems, final loss {loss:.4f}")
-    del Ut, dbt, dxt
+    del Ut
+    sel = tune.Selector(seed=0)
+    loss = sel.fit_multi(feats_t, times_t, list(ms), epochs=800)
+    progress(f"batch: selector trained on {train_count} systems x m in {list(ms)}, final loss {loss:.4f}")
     U, sp, kinds = gen.batch_systems(count, seed=3, nmin=256, nmax=nmax)
     B = capi.Batch(U, sp)
-    _, b = gen.rhs_for(U, m, seed=1, layout="row")
-    d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
-    T = tune.exhaustive(B, d_b, d_x, m=m, reps=2)
-    feats = B.features(m)
-    choices = {"fallback_ls1": np.full(B.nsys, capi.BCFG_LS1), "exhaustive": np.argmin(T, axis=1),
-               "mlp": sel.predict(feats, m=m)}
-    res = {"systems": count, "n_total": U.n, "nnz_total": U.nnz, "m": m, "nmax": nmax,
-           "train_systems": train_count, "train_final_loss": loss}
-    for name, ch in choices.items():
-        cost = T[np.arange(B.nsys), ch]
-        B.set_configs(ch, tune.lpt_order(cost))
-        B.solve(d_b, d_x, m, capi.FP64, True)
-        B.sync()
-        ts = []
-        for _ in range(5):
-            t0 = time.perf_counter()
+    res = {"systems": count, "n_total": U.n, "nnz_total": U.nnz, "nmax": nmax, "train_systems": train_count,
+           "train_final_loss": loss}
+    for m in ms:
+        _, b = gen.rhs_for(U, m, seed=1, layout="row")
+        d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+        T = tune.exhaustive(B, d_b, d_x, m=m, reps=2)
+        feats = B.features(m)
+        choices = {"fallback_ls1": np.full(B.nsys, capi.BCFG_LS1), "exhaustive": np.argmin(T, axis=1),
+                   "mlp": sel.predict(feats, m=m)}
+        rm = {}
+        for name, ch in choices.items():
+            cost = T[np.arange(B.nsys), ch]
+            B.set_configs(ch, tune.lpt_order(cost))
             B.solve(d_b, d_x, m, capi.FP64, True)
             B.sync()
-            ts.append(time.perf_counter() - t0)
-        best = min(ts)
-        res[name] = {"ms": round(best * 1e3, 3), "gnnz_per_s": round(U.nnz * m / best / 1e9, 3),
-                     "sum_device_us": round(float(cost.sum()), 1), "regret": round(tune.regret(T, ch), 4)}
-        progress(f"batch {name}: {res[name]}")
-    res["mlp_vs_exhaustive_throughput"] = round(res["exhaustive"]["ms"] / res["mlp"]["ms"], 4)
-    res["mlp_accuracy"] = float((choices["mlp"] == choices["exhaustive"]).mean())
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                B.solve(d_b, d_x, m, capi.FP64, True)
+                B.sync()
+                ts.append(time.perf_counter() - t0)
+            best = min(ts)
+            rm[name] = {"ms": round(best * 1e3, 3), "gnnz_per_s": round(U.nnz * m / best / 1e9, 3),
+                        "sum_device_us": round(float(cost.sum()), 1), "regret": round(tune.regret(T, ch), 4)}
+        rm["mlp_vs_exhaustive_throughput"] = round(rm["exhaustive"]["ms"] / rm["mlp"]["ms"], 4)
+        rm["mlp_accuracy"] = float((choices["mlp"] == choices["exhaustive"]).mean())
+        # FP32 solves + FP64 refinement with the selector's configurations
+        ch = choices["mlp"]
+        B.set_configs(ch, tune.lpt_order(T[np.arange(B.nsys), ch]))
+        B.solve_refined(d_b, d_x, m)
+        t0 = time.perf_counter()
+        it, est = B.solve_refined(d_b, d_x, m)
+        t_ref = time.perf_counter() - t0
+        rm["fp32_refined"] = {"ms": round(t_ref * 1e3, 3), "iters": it, "error_estimate": est}
+        res[f"m{m}"] = rm
+        progress(f"batch m={m}: {rm}")
+        d_b.free(); d_x.free()
     out["batch"] = res
     sel.close()
     B.close()
@@ -233,6 +261,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run N-byte device copies (counter calibration)")
     args = ap.parse_args()
+    t_start = time.perf_counter()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the solve path here is single-GPU "
               "(multi-GPU sharding is design-only for this build)", file=sys.stderr)
@@ -271,7 +300,9 @@ def main():
     if not args.no_extras:
         runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
                 ("ic0_128", "level_pull", 32), ("ic0_128", "level_pull", 128), ("zipf_1e6", "tile", 1),
-                ("zipf_1e6", "chain", 1), ("chol2d_100", "chain", 1), ("chol2d_100", "level_pull", 1)]
+                ("zipf_1e6", "chain", 1), ("chol2d_100", "chain", 1), ("chol2d_100", "level_pull", 1),
+                ("supernodal_4096", "supernodal", 8), ("supernodal_4096", "supernodal", 32),
+                ("supernodal_4096", "supernodal", 128)]
         cache = {args.workload: (U, P)}
         for wl, vname, mm in runs:
             if (wl, vname, mm) == (args.workload, args.variant, m):
@@ -280,6 +311,8 @@ def main():
                 if wl not in cache:
                     UU, _ = make_workload(wl)
                     cache[wl] = (UU, capi.Plan.from_upper(UU))
+                    if wl.startswith("supernodal"):
+                        cache[wl][1].build_supernodes(UU.snodes)
                 UU, PP = cache[wl]
                 _, bx = gen.rhs_for(UU, mm, seed=1, layout="row")
                 db = capi.DeviceArray.from_host(bx)
@@ -294,8 +327,19 @@ def main():
                     cpu_ms = {f"{wl}/cpu_serial_ms": round(t_cpu * 1e3, 3)}
                     extras.update(cpu_ms)
                     e_err = O.rel_error(ref_w, xg)
+                elif wl.startswith("supernodal") and mm == 8:
+                    # CPU baselines on the same factor (one host core each, one solve): the serial
+                    # solver and the reference's supernodal method, restated in the oracle
+                    bcol = np.ascontiguousarray(bx.T)
+                    ref_w, t_ser = O.base_gauss_upper(UU.rowptr, UU.col, UU.val, bcol)
+                    _, t_sn = O.supernodal_upper(UU.rowptr, UU.col, UU.val, UU.snodes, bcol, new=True)
+                    extras[f"{wl}/m{mm}/cpu_serial_ms"] = round(t_ser * 1e3, 3)
+                    extras[f"{wl}/m{mm}/cpu_supernodal_ms"] = round(t_sn * 1e3, 3)
+                    e_err = O.rel_error(np.ascontiguousarray(ref_w.T), xg)
                 else:
                     e_err = None
+                if vname == "supernodal":
+                    extras[f"{wl}/{vname}/m{mm}/TFLOP_s"] = round(supernodal_flops(UU, mm) / (e_kms / 1e3) / 1e12, 3)
                 extras[f"{wl}/{vname}/m{mm}"] = {"ms_per_step": round(e_ms, 4), "kernel_ms": round(e_kms, 4),
                                                  "rel_err_vs_oracle": e_err,
                                                  "GB_s": round(BB / (e_ms / 1e3) / 1e9, 2),
@@ -330,6 +374,17 @@ def main():
             bench_batch(capi, gen, batch)
         except Exception as exc:
             batch["error"] = str(exc)
+    if variant == capi.VAR_SUPERNODAL:
+        flops = supernodal_flops(U, m)
+        tfs = flops / (kms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(tfs / FP64_MFMA_PEAK_TFS, 5), "traffic": read_traffic(args.workload, args.variant, m),
+                "kernel_ms": round(kms, 5), "alg_flops": flops, "alg_bytes": B,
+                "hbm_GB_s": round(achieved, 3)}
+    else:
+        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args.workload, args.variant, m),
+                "kernel_ms": round(kms, 5), "alg_bytes": B}
     out = {
         "metric": "SpTRSV GB/s & % HBM peak; time-to-1e-12 error, mixed+refine vs FP64 sync-free",
         "value": round(gbs, 3),
@@ -349,13 +404,12 @@ def main():
         "nnz_per_s": U.nnz * m / (ms / 1e3),
         "rel_err_vs_oracle": err,
         "warmup_solves_ms": first,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args.workload, args.variant, m),
-                     "kernel_ms": round(kms, 5), "alg_bytes": B},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "extras": extras,
         "mixed_refine": mixed,
         "batched": batch,
+        "bench_wall_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(out))
 

@@ -189,6 +189,11 @@ int sptrsv_batch_features(sptrsv_batch_t batch, int m, float* feats);
 int sptrsv_batch_set_configs(sptrsv_batch_t batch, const int* cfg, const int* order);
 int sptrsv_batch_solve(sptrsv_batch_t batch, int precision, const double* d_b, double* d_x, int m, int use_graph);
 int sptrsv_batch_sync(sptrsv_batch_t batch);
+/* FP32 batched solves with FP64 refinement (residual over all systems, stopping rule of
+ * sptrsv_solve_refined over the inf-norms of all systems together; opts->precision FP32 or 0).
+ * Synchronous; *estimate = estimated relative error of x. */
+int sptrsv_batch_solve_refined(sptrsv_batch_t batch, const sptrsv_refine_t* opts, const double* d_b, double* d_x,
+                               int m, int* iters, double* estimate);
 /* The block-diagonal plan under the batch (for residuals, timing, refinement). */
 sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t batch);
 /* Autotuner primitive: every system on `cfg` in one launch; device microseconds per system. */

@@ -110,6 +110,8 @@ def lib():
         "sptrsv_batch_set_configs": (_i, [_vp, _vp, _vp]),
         "sptrsv_batch_solve": (_i, [_vp, _i, _vp, _vp, _i, _i]),
         "sptrsv_batch_sync": (_i, [_vp]),
+        "sptrsv_batch_solve_refined": (_i, [_vp, ctypes.POINTER(RefineOpts), _vp, _vp, _i, ctypes.POINTER(_i),
+                                            ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_batch_plan": (_vp, [_vp]),
         "sptrsv_batch_time_config": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp]),
         "sptrsv_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_uint, ctypes.POINTER(_vp)]),
@@ -415,6 +417,15 @@ class Batch:
 
     def sync(self):
         check(lib().sptrsv_batch_sync(self.handle), "batch_sync")
+
+    def solve_refined(self, d_b, d_x, m=1, tol=0.0, max_iters=0):
+        """FP32 batched solves + FP64 refinement (synchronous): (corrections, error estimate)."""
+        o = RefineOpts()
+        o.max_iters, o.precision, o.tol = max_iters, FP32, tol
+        it, est = _i(0), ctypes.c_double(0.0)
+        check(lib().sptrsv_batch_solve_refined(self.handle, ctypes.byref(o), d_b.ptr, d_x.ptr, int(m), ctypes.byref(it),
+                                               ctypes.byref(est)), "batch_solve_refined")
+        return it.value, est.value
 
     def time_config(self, cfg, d_b, d_x, m=1, precision=FP64):
         us = np.zeros(self.nsys, dtype=np.float64)

@@ -488,6 +488,24 @@ int sptrsv_batch_solve(sptrsv_batch_t B, int precision, const double* d_b, doubl
     return SPTRSV_OK;
 }
 
+// FP32 batched solves with FP64 refinement: the residual runs over the whole block-diagonal
+// CSR, the stopping rule over the inf-norms of all systems together.  Synchronous.
+int sptrsv_batch_solve_refined(sptrsv_batch_t B, const sptrsv_refine_t* opts, const double* d_b, double* d_x, int m,
+                               int* iters, double* estimate) {
+    if (!B || !d_b || !d_x || m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "bad batch solve arguments");
+    sptrsv_refine_t ro{};
+    if (opts) ro = *opts;
+    if (ro.precision != 0 && ro.precision != SPTRSV_FP32)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "batched refinement runs FP32 inner solves");
+    sptrsv_plan_s* P = B->plan;
+    SPT_CHECK(ensure_fp32(P));
+    auto solve_low = [&](const float* in, float* out) -> int {
+        return launch_batch<float>(B, P->val32, in, out, m, nullptr);
+    };
+    auto residual = [&](const double* b, const double* x, float* r) -> int { return residual32(P, b, x, r, m); };
+    return refine_loop(P, ro, d_b, d_x, m, solve_low, residual, iters, estimate);
+}
+
 int sptrsv_batch_sync(sptrsv_batch_t B) {
     if (!B) return fail(SPTRSV_ERR_BAD_INPUT, "null batch");
     return sptrsv_plan_sync(B->plan);

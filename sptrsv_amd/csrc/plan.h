@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <utility>
 #include <vector>
 
@@ -109,6 +110,11 @@ int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double*
                      uint64_t limit);  // supernodal.hip
 void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip
 int supernodal_residual32(sptrsv_plan_s* p, const double* d_b, const double* d_x, float* d_r32, int m);
+int residual32(sptrsv_plan_s* p, const double* d_b, const double* d_x, float* d_r32, int m);
+int refine_loop(sptrsv_plan_s* p, const sptrsv_refine_t& ro, const double* d_b, double* d_x, int m,
+                const std::function<int(const float*, float*)>& solve_low,
+                const std::function<int(const double*, const double*, float*)>& residual, int* iters,
+                double* ratio);
 int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
                          uint64_t limit);  // supernodal.hip
 int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -596,6 +597,74 @@ int default_variant(const sptrsv_plan_s* P, int m) {
     return SPTRSV_VAR_LEVEL_PULL;
 }
 
+// r = b - U x over the plan's CSR in FP64, stored FP32 (any factor; one thread per (row, rhs)).
+int residual32(sptrsv_plan_s* P, const double* d_b, const double* d_x, float* d_r32, int m) {
+    const int64_t nm = (int64_t)P->n * m;
+    k_residual32<<<grid_for(nm), 256, 0, P->stream>>>(P->n, m, P->rowptr, P->col, P->val, d_x, d_b, d_r32);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+// FP64 iterative refinement around a reduced-precision solve (see sptrsv_solve_refined).
+// Stopping rule: correction k has relative size r_k ~ the error of x before it; with the
+// observed contraction c = r_k / r_{k-1} the error left after it is about r_k * c.  Stop when
+// that estimate is <= tol, or when the corrections stop shrinking (the FP64 residual floor).
+int refine_loop(sptrsv_plan_s* P, const sptrsv_refine_t& ro, const double* d_b, double* d_x, int m,
+                const std::function<int(const float*, float*)>& solve_low,
+                const std::function<int(const double*, const double*, float*)>& residual, int* iters,
+                double* ratio) {
+    const int max_iters = ro.max_iters > 0 ? ro.max_iters : 30;
+    const double tol = ro.tol > 0 ? ro.tol : 1e-13;
+    const int n = P->n;
+    const int64_t nm = (int64_t)n * m;
+    hipStream_t s = P->stream;
+    if ((int64_t)P->ref_elems < nm || (int)P->ref_cols < m) {
+        if (P->ref_r32) SPT_HIP(hipFree(P->ref_r32));
+        if (P->ref_d32) SPT_HIP(hipFree(P->ref_d32));
+        if (P->ref_norms) SPT_HIP(hipFree(P->ref_norms));
+        P->ref_r32 = P->ref_d32 = nullptr;
+        P->ref_norms = nullptr;
+        SPT_HIP(hipMalloc((void**)&P->ref_r32, nm * sizeof(float)));
+        SPT_HIP(hipMalloc((void**)&P->ref_d32, nm * sizeof(float)));
+        SPT_HIP(hipMalloc((void**)&P->ref_norms, 2 * (size_t)m * sizeof(unsigned long long)));
+        P->ref_elems = (size_t)nm;
+        P->ref_cols = (size_t)m;
+    }
+    const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));
+    std::vector<unsigned long long> h((size_t)2 * m);
+    k_convert<double, float><<<grid_for(nm), 256, 0, s>>>(d_b, P->ref_r32, nm);
+    SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
+    k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
+    k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 1);
+    SPT_HIP(hipGetLastError());
+    SPT_CHECK(sptrsv_plan_sync(P));
+    double last = 1.0, prev = 1.0, estimate = 1.0;
+    int done = 0;
+    for (int it = 1; it <= max_iters; ++it) {
+        SPT_CHECK(residual(d_b, d_x, P->ref_r32));
+        SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
+        k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
+        k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 0);
+        SPT_HIP(hipGetLastError());
+        SPT_CHECK(sptrsv_plan_sync(P));
+        SPT_HIP(hipMemcpy(h.data(), P->ref_norms, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        last = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const double dn = __builtin_bit_cast(double, h[(size_t)k]);
+            const double xn = __builtin_bit_cast(double, h[(size_t)m + k]);
+            last = std::max(last, xn > 0 ? dn / xn : dn);
+        }
+        done = it;
+        estimate = last * std::min(1.0, last / prev);
+        if (!(estimate > tol)) break;
+        if (it >= 2 && last > 0.5 * prev) break;
+        prev = last;
+    }
+    if (iters) *iters = done;
+    if (ratio) *ratio = estimate;
+    return SPTRSV_OK;
+}
+
 }  // namespace sptrsv
 
 using namespace sptrsv;
@@ -688,8 +757,6 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     if (params) prm = *params;
     sptrsv_refine_t ro{};
     if (opts) ro = *opts;
-    const int max_iters = ro.max_iters > 0 ? ro.max_iters : 30;
-    const double tol = ro.tol > 0 ? ro.tol : 1e-13;
     const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
     int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
     if (variant == SPTRSV_VAR_TILE)
@@ -702,66 +769,18 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
                                                : P->ticks_per_ms * 20000ull;
     const int unroll = prm.unroll > 0 ? prm.unroll : 4;
-    const int n = P->n;
-    const int64_t nm = (int64_t)n * m;
-    hipStream_t s = P->stream;
-    if ((int64_t)P->ref_elems < nm || (int)P->ref_cols < m) {
-        if (P->ref_r32) SPT_HIP(hipFree(P->ref_r32));
-        if (P->ref_d32) SPT_HIP(hipFree(P->ref_d32));
-        if (P->ref_norms) SPT_HIP(hipFree(P->ref_norms));
-        P->ref_r32 = P->ref_d32 = nullptr;
-        P->ref_norms = nullptr;
-        SPT_HIP(hipMalloc((void**)&P->ref_r32, nm * sizeof(float)));
-        SPT_HIP(hipMalloc((void**)&P->ref_d32, nm * sizeof(float)));
-        SPT_HIP(hipMalloc((void**)&P->ref_norms, 2 * (size_t)m * sizeof(unsigned long long)));
-        P->ref_elems = (size_t)nm;
-        P->ref_cols = (size_t)m;
-    }
     if (variant != SPTRSV_VAR_SUPERNODAL) SPT_CHECK(ensure_fp32(P));
     auto solve_low = [&](const float* in, float* out) -> int {
         if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve_low(P, low, in, out, m, limit);
         if (variant == SPTRSV_VAR_CHAIN) return chain_solve32(P, in, out);
-        return solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, in, out, m, P->val32, P->cval32, P->lval32, limit);
+        return solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, in, out, m, P->val32,
+                                  P->cval32, P->lval32, limit);
     };
-    const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));
-    std::vector<unsigned long long> h((size_t)2 * m);
-    k_convert<double, float><<<grid_for(nm), 256, 0, s>>>(d_b, P->ref_r32, nm);
-    SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
-    k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
-    k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 1);
-    SPT_HIP(hipGetLastError());
-    SPT_CHECK(sptrsv_plan_sync(P));
-    // Stopping rule: correction k has relative size r_k ~ the error of x before it; with the
-    // observed contraction c = r_k / r_{k-1} the error left after it is about r_k * c.  Stop when
-    // that estimate is <= tol, or when the corrections stop shrinking (the FP64 residual floor).
-    double last = 1.0, prev = 1.0, estimate = 1.0;
-    int done = 0;
-    for (int it = 1; it <= max_iters; ++it) {
-        if (variant == SPTRSV_VAR_SUPERNODAL)
-            SPT_CHECK(supernodal_residual32(P, d_b, d_x, P->ref_r32, m));
-        else
-            k_residual32<<<grid_for(nm), 256, 0, s>>>(n, m, P->rowptr, P->col, P->val, d_x, d_b, P->ref_r32);
-        SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
-        k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
-        k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 0);
-        SPT_HIP(hipGetLastError());
-        SPT_CHECK(sptrsv_plan_sync(P));
-        SPT_HIP(hipMemcpy(h.data(), P->ref_norms, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        last = 0.0;
-        for (int k = 0; k < m; ++k) {
-            const double dn = __builtin_bit_cast(double, h[(size_t)k]);
-            const double xn = __builtin_bit_cast(double, h[(size_t)m + k]);
-            last = std::max(last, xn > 0 ? dn / xn : dn);
-        }
-        done = it;
-        estimate = last * std::min(1.0, last / prev);
-        if (!(estimate > tol)) break;
-        if (it >= 2 && last > 0.5 * prev) break;
-        prev = last;
-    }
-    if (iters) *iters = done;
-    if (ratio) *ratio = estimate;
-    return SPTRSV_OK;
+    auto residual = [&](const double* b, const double* x, float* r) -> int {
+        if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_residual32(P, b, x, r, m);
+        return residual32(P, b, x, r, m);
+    };
+    return refine_loop(P, ro, d_b, d_x, m, solve_low, residual, iters, ratio);
 }
 
 int sptrsv_residual(sptrsv_plan_t P, const double* d_b, const double* d_x, double* d_r, int m) {

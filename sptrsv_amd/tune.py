@@ -60,6 +60,16 @@ class Selector:
         y = np.argmin(times, axis=1).astype(np.int32)
         return self.mlp.train(self._norm(feats), y, valid_mask(len(y), m), epochs=epochs, lr=lr)
 
+    def fit_multi(self, feats: list, times: list, ms: list, epochs: int = 600, lr: float = 5e-3) -> float:
+        """One network over several RHS counts (the m feature tells them apart): features and
+        autotuner tables per m, concatenated; configurations invalid at an m are masked."""
+        F = np.concatenate([np.asarray(f, dtype=np.float32) for f in feats])
+        y = np.concatenate([np.argmin(t, axis=1) for t in times]).astype(np.int32)
+        mask = np.concatenate([valid_mask(len(t), m) for t, m in zip(times, ms)])
+        self.mu = F.mean(axis=0)
+        self.sd = F.std(axis=0) + 1e-6
+        return self.mlp.train(self._norm(F), y, mask, epochs=epochs, lr=lr)
+
     def predict(self, feats: np.ndarray, m: int = 1) -> np.ndarray:
         logits = self.mlp.forward(self._norm(feats))
         logits[valid_mask(len(logits), m) == 0] = -np.inf

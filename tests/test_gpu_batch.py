@@ -87,6 +87,23 @@ def test_batch_graph_replays(small_batch):
         assert O.rel_error(oracle_rowmajor(U, b), d_x.download()) <= FP64_TOL, rep
 
 
+@pytest.mark.parametrize("m", [1, 4, 16])
+def test_batch_refined_reaches_fp64(small_batch, m):
+    """FP32 batched solves + FP64 refinement: every system within 1e-12 of the oracle."""
+    U, sp, kinds, B = small_batch
+    B.set_configs(np.full(B.nsys, capi.BCFG_LS4), np.arange(B.nsys))
+    xt, b = gen.rhs_for(U, m, seed=8, layout="row")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    it, est = B.solve_refined(d_b, d_x, m)
+    x = d_x.download()
+    ref = oracle_rowmajor(U, b)
+    assert 1 <= it <= 4 and est <= 1e-13, (it, est)
+    for s in range(B.nsys):
+        lo, hi = sp[s], sp[s + 1]
+        assert O.rel_error(ref[lo:hi], x[lo:hi]) <= FP64_TOL, (s, kinds[s])
+
+
 def test_batch_rejects_cross_system_columns():
     U = gen.random_zipf(100, seed=1)
     with pytest.raises(capi.SptrsvError) as e:

@@ -176,7 +176,7 @@ int sptrsv_solve_refined(sptrsv_plan_t plan, const sptrsv_params_t* params, cons
 #define SPTRSV_BCFG_LS4 1   /* 4 lanes per row (entries when m == 1, RHS columns otherwise) */
 #define SPTRSV_BCFG_LS16 2
 #define SPTRSV_BCFG_LS64 3
-#define SPTRSV_BCFG_CHAIN 4 /* one wave, 64-row windows (m == 1) */
+#define SPTRSV_BCFG_CHAIN 4 /* 64-row windows in order (m <= 32; m > 1: waves over the RHS) */
 #define SPTRSV_BCFG_COUNT 5
 typedef struct sptrsv_batch_s* sptrsv_batch_t;
 int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t* rowptr, const int* col,

@@ -6,7 +6,8 @@
 //   LS_G1 / LS_G4 / LS_G16 / LS_G64   one workgroup walks the system's levels with a barrier
 //       between levels; G lanes per row (over the right-hand sides when m > 1, over the row's
 //       entries when m == 1 -- then the partial sums meet through cross-lane shuffles)
-//   CHAIN   (m == 1) wave 0 runs the 64-row window recurrence of chain.hip on the system
+//   CHAIN   (m <= 32) the 64-row window recurrence of chain.hip: m == 1 on wave 0 alone,
+//       m > 1 with the window tile shared in LDS and the waves taking the right-hand sides
 // Persistent workgroups (2 per CU) pop systems from a queue in a given order (longest
 // predicted time first) until it is empty; systems never wait on each other, so there is no
 // cross-workgroup dependency at all.  With a timing buffer the kernel also records every
@@ -47,7 +48,8 @@ struct sptrsv_batch_s {
 namespace sptrsv {
 namespace {
 
-constexpr int kBRing = 2048;  // x rows kept in LDS by the chain configuration
+constexpr int kBRing = 2048;     // x values kept in LDS by the chain configuration
+constexpr int kBChainMaxM = 32;  // chain configuration up to this many right-hand sides
 
 template <typename T>
 struct BatchArgs {
@@ -123,11 +125,71 @@ __device__ void chain_system(const BatchArgs<T>& a, int lo, int top, T (*tile)[6
     wait_vmem();
 }
 
+// The same window recurrence with m right-hand sides (m <= 32): the window's coefficient tile
+// is built once by wave 0 and shared; wave w solves columns w, w + 4, ...; the LDS ring holds
+// the last kBRing / m rows of each column (>= 64, so the previous window is always in it).
+template <typename T>
+__device__ void chain_system_m(const BatchArgs<T>& a, int lo, int top, T (*tile)[64], T* ring, T* sinv) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = a.m;
+    int R = kBRing;
+    while (R * m > kBRing) R >>= 1;
+    for (int hi = top; hi > lo; hi -= 64) {
+        const int r = hi - 1 - lane;
+        const bool valid = r >= lo;
+        if (wave == 0) {
+#pragma unroll 8
+            for (int s = 0; s < 64; ++s) tile[s][lane] = T(0);
+            T diag = T(1);
+            if (valid) {
+                const int st = a.rowptr[r], en = a.rowptr[r + 1];
+                diag = a.val[st];
+                for (int p = st + 1; p < en; ++p) {
+                    const int c = a.col[p];
+                    if (c < hi) tile[hi - 1 - c][lane] = a.val[p];
+                }
+            }
+            sinv[lane] = T(1) / diag;
+        }
+        __syncthreads();
+        const T inv = sinv[lane];
+        for (int k = wave; k < m; k += 4) {
+            T* rk = ring + k * R;
+            T acc = T(0), bv = T(0);
+            if (valid) {
+                const int st = a.rowptr[r], en = a.rowptr[r + 1];
+                bv = a.b[(size_t)r * m + k];
+                for (int p = st + 1; p < en; ++p) {
+                    const int c = a.col[p];
+                    if (c < hi) continue;
+                    const T v = a.val[p];
+                    acc += v * (c < hi + R ? rk[c & (R - 1)] : a.x[(size_t)c * m + k]);
+                }
+            }
+            const T binv = bv * inv;
+            T mine = T(0);
+#pragma unroll 16
+            for (int s = 0; s < 64; ++s) {
+                const T xs = fma(-acc, inv, binv);
+                const T xb = bcast_b(xs, s);
+                if (lane == s) mine = xs;
+                acc += tile[s][lane] * xb;
+            }
+            if (valid) {
+                rk[r & (R - 1)] = mine;
+                a.x[(size_t)r * m + k] = mine;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs<T> a) {
 #pragma clang fp contract(off)
     __shared__ T tile[64][64];
     __shared__ T ring[kBRing];
+    __shared__ T sinv[64];
     __shared__ int s_sys;
     const int tid = threadIdx.x;
     const size_t m = (size_t)a.m;
@@ -145,6 +207,8 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs<T> a) {
         if (a.timer && tid == 0) t0 = wall_clock64();
         if (cfg == SPTRSV_BCFG_CHAIN && a.m == 1) {
             if (tid < 64) chain_system(a, lo, hi, tile, ring);
+        } else if (cfg == SPTRSV_BCFG_CHAIN && a.m <= kBChainMaxM) {
+            chain_system_m(a, lo, hi, tile, ring, sinv);
         } else {
             const int G = cfg == SPTRSV_BCFG_LS1 ? 1 : cfg == SPTRSV_BCFG_LS4 ? 4 : cfg == SPTRSV_BCFG_LS16 ? 16 : 64;
             const int groups = 256 / G, grp = tid / G, gl = tid % G;
@@ -514,7 +578,7 @@ int sptrsv_batch_sync(sptrsv_batch_t B) {
 sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t B) { return B ? B->plan : nullptr; }
 
 // Autotuner primitive: one launch with every system on configuration `cfg` (systems for which
-// cfg is invalid -- CHAIN with m > 1 -- run LS_G1), device time of each system in microseconds.
+// cfg is invalid -- CHAIN with m > 32 -- run LS_G1), device time of each system in microseconds.
 int sptrsv_batch_time_config(sptrsv_batch_t B, int cfg, int precision, const double* d_b, double* d_x, int m,
                              double* sys_us) {
     if (!B || !sys_us || cfg < 0 || cfg >= SPTRSV_BCFG_COUNT) return fail(SPTRSV_ERR_BAD_INPUT, "bad timing arguments");

@@ -13,11 +13,12 @@ import numpy as np
 from . import capi
 
 K = capi.BCFG_COUNT
+CHAIN_MAX_M = 32  # the chain configuration serves up to this many right-hand sides
 
 
 def valid_mask(nsys: int, m: int) -> np.ndarray:
     mask = np.ones((nsys, K), dtype=np.float32)
-    if m > 1:
+    if m > CHAIN_MAX_M:
         mask[:, capi.BCFG_CHAIN] = 0.0
     return mask
 

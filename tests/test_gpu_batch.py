@@ -41,17 +41,15 @@ def solve_batch(B, b, m, precision=capi.FP64, graph=False):
 
 
 @pytest.mark.parametrize("cfg", range(capi.BCFG_COUNT))
-@pytest.mark.parametrize("m", [1, 4, 16])
+@pytest.mark.parametrize("m", [1, 4, 16, 32])
 def test_batch_every_config(small_batch, cfg, m):
     U, sp, kinds, B = small_batch
-    if cfg == capi.BCFG_CHAIN and m > 1:
-        pytest.skip("chain configuration is m == 1 (the kernel falls back to LS1)")
     banner(f"batch cfg={capi.BCFG_NAMES[cfg]} m={m}")
     B.set_configs(np.full(B.nsys, cfg), np.arange(B.nsys)[::-1])
     xt, b = gen.rhs_for(U, m, seed=4, layout="row")
     x = solve_batch(B, b, m)
     ref = oracle_rowmajor(U, b)
-    exact = (cfg == capi.BCFG_LS1) or m > 1
+    exact = cfg != capi.BCFG_CHAIN and (cfg == capi.BCFG_LS1 or m > 1)
     if exact:
         assert np.array_equal(x, ref), O.rel_error(ref, x)
     for s in range(B.nsys):  # every system on its own
@@ -126,6 +124,11 @@ def test_time_config_and_exhaustive(small_batch):
     d_b = capi.DeviceArray.from_host(b)
     d_x = capi.DeviceArray(b.shape)
     T = tune.exhaustive(B, d_b, d_x, m=4, reps=1)
+    assert np.all(np.isfinite(T)) and np.all(T > 0)
+    xt, b = gen.rhs_for(U, 40, seed=6, layout="row")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(b.shape)
+    T = tune.exhaustive(B, d_b, d_x, m=40, reps=1)  # beyond the chain configuration's m
     assert np.all(np.isinf(T[:, capi.BCFG_CHAIN]))
     assert np.all(np.isfinite(T[:, :capi.BCFG_CHAIN])) and np.all(T[:, :capi.BCFG_CHAIN] > 0)
 

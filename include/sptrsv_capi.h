@@ -50,10 +50,11 @@ extern "C" {
 #define SPTRSV_VAR_SUPERNODAL 6   /* one workgroup per supernode: dense blocks on MFMA, the
                                      diagonal block applied through its precomputed inverse
                                      (needs sptrsv_plan_build_supernodes) */
-#define SPTRSV_VAR_TILE 7         /* one 1024-thread workgroup per tile of <= 960 consecutive rows:
-                                     entries staged in LDS, dependencies inside the tile resolved
-                                     through LDS, outside ones fetched by a poller wave; m == 1,
-                                     exact reference summation order */
+#define SPTRSV_VAR_TILE 7         /* one workgroup per tile of T consecutive rows (params.block;
+                                     0 = 16384) with the tile's x in LDS: the tile's rows run level
+                                     by level with a barrier between levels, dependencies inside the
+                                     tile read from LDS, later tiles polled; m == 1, FP64, exact
+                                     reference summation order */
 #define SPTRSV_VAR_COUNT 8
 
 /* Value precision of the solve. */
@@ -72,7 +73,8 @@ typedef struct {
                          hardware limit; 0 = default: LEVEL_PULL about three mean level widths of
                          rows in flight, ROW_PULL the hardware limit */
     int block;        /* pull kernels: threads per workgroup (64..1024, power of two); 0 =
-                         default: LEVEL_PULL 64 rows per workgroup, ROW_PULL 256 / 1024 threads */
+                         default: LEVEL_PULL 64 rows per workgroup, ROW_PULL 256 / 1024 threads.
+                         TILE: rows per tile (1..16384; 0 = 16384) */
     int reserved[1];
 } sptrsv_params_t;
 

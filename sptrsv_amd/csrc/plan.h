@@ -84,15 +84,22 @@ struct sptrsv_plan_s {
     size_t ref_elems = 0;
     size_t ref_cols = 0;
 
-    // tiles for the TILE variant (built on first use): tile t covers rows [tile_lo[t],
-    // tile_hi[t]) (tile 0 = last rows); tile_ext[tile_ext_ptr[t]..] are the columns outside the
-    // tile that its rows read (descending); lidx[p] is the LDS slot of CSR entry p.
+    // tiles for the TILE variant (built on first use for a tile size): tile t = rows
+    // [t T, (t+1) T); its steps are tile_step[tile_sptr[t] .. tile_sptr[t+1]) (first positions);
+    // position p is the 80-byte record tile_rec[p] (row, length, first entries, diagonal),
+    // further entries in tile_col / tile_val; tile_rlist: per step, tile_cap (remote column,
+    // ring slot) pairs for the poller wave.
     int n_tiles = 0;
-    int* tile_lo = nullptr;
-    int* tile_hi = nullptr;
-    int* tile_ext_ptr = nullptr;
-    int* tile_ext = nullptr;
-    uint16_t* tile_lidx = nullptr;
+    int tile_rows = 0;
+    int tile_cap = 0;
+    int* tile_sptr = nullptr;
+    int* tile_step = nullptr;
+    void* tile_rec = nullptr;
+    int* tile_col = nullptr;
+    double* tile_val = nullptr;
+    void* tile_rlist = nullptr;
+    int* tile_row = nullptr;      // row of each position
+    double* tile_bt = nullptr;    // b in position order (per solve)
 
     hipStream_t stream = nullptr;
     bool timing = false;
@@ -117,7 +124,7 @@ int refine_loop(sptrsv_plan_s* p, const sptrsv_refine_t& ro, const double* d_b, 
                 double* ratio);
 int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
                          uint64_t limit);  // supernodal.hip
-int tile_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
+int tile_solve(sptrsv_plan_s* p, int precision, int rows, const double* d_b, double* d_x, int m,
                uint64_t limit);  // tile.hip
 void tile_free(sptrsv_plan_s* p);         // tile.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip

@@ -683,7 +683,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
                                             : P->ticks_per_ms * 20000ull;
     const int unroll = prm.unroll > 0 ? prm.unroll : 4;
     if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
-    if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, d_b, d_x, m, limit);
+    if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, prm.block, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_CHAIN) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, prm.occupancy, prm.block, d_b, d_x, m, P->val, P->cval, P->lval, limit);

@@ -292,6 +292,20 @@ def test_full_size_zipf_1e6_level_pull():
         assert O.rel_error(ref, got) <= FP64_TOL
 
 
+@pytest.mark.parametrize("rows", [1, 64, 1000, 1024, 4096])
+def test_tile_sizes(plans, rows):
+    """Tiles of `rows` consecutive rows: many tiles (cross-tile polling), ragged last tiles and
+    single-row tiles; FP64 x stays bitwise equal to the oracle."""
+    for name in ("zipf_3000", "ic0_12", "chain_4000", "chol2d_15", "snode_10"):
+        U, P = MATS[name], plans[name]
+        banner(f"tile rows={rows} {name}")
+        xt, b = gen.rhs_for(U, 1, seed=23)
+        x = P.solve_host(b, capi.params(variant=capi.VAR_TILE, block=rows))
+        assert np.array_equal(x, oracle_solve(U, b)), (name, rows)
+    with pytest.raises(capi.SptrsvError):
+        P.solve_host(b, capi.params(variant=capi.VAR_TILE, block=20000))
+
+
 def test_tile_rejects_multi_rhs(plans):
     U, P = MATS["zipf_3000"], plans["zipf_3000"]
     xt, b = gen.rhs_for(U, 2, seed=1, layout="row")

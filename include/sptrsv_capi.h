@@ -65,7 +65,9 @@ extern "C" {
 
 typedef struct {
     int variant;      /* SPTRSV_VAR_* */
-    int group;        /* lanes per row for the pull kernels (1,2,4,...,64); 0 = derive from m */
+    int group;        /* lanes per row for the pull kernels (1,2,4,...,64); 0 = derive from m
+                         (m lanes up to 64). m == 1: lanes splitting a row's entries; LEVEL_PULL
+                         with m > 1: 8..64 lanes, each taking up to 8 columns */
     int unroll;       /* entries polled per step in the pull kernels (1, 2 or 4); 0 = default */
     int precision;    /* SPTRSV_FP64 / SPTRSV_FP32 */
     int watchdog_us;  /* per-wait device watchdog in microseconds; 0 = default (20 s) */

@@ -441,6 +441,8 @@ template <typename T, bool LEVELED>
 int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int unroll, int occ, int block) {
     // lanes per row: m == 1 -> 1 (or `group` lanes splitting entries); m <= 64 -> next pow2 of
     // m; m > 64 -> 64 lanes with up to 4 columns each, launched in chunks of 256 columns.
+    // LEVEL_PULL with m > 1 takes `group` (8..64) as the lanes per row instead: fewer lanes
+    // per row put more rows in flight per wave, each lane then takes up to 8 columns.
     hipStream_t s = P->stream;
     const int n = P->n;
     if (m == 1 && group > 1) {
@@ -469,8 +471,12 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
         PullArgs<T> a = base;
         a.k0 = k0;
         a.kc = kc;
-        const int G = kc == 1 ? 1 : std::min(64, next_pow2(kc));
-        const int kpl = (kc + G - 1) / G;  // 1..4
+        int G = kc == 1 ? 1 : std::min(64, next_pow2(kc));
+        if (LEVELED && kc > 1 && group > 0) {
+            G = std::min(G, std::max(8, next_pow2(std::min(group, 64))));
+            while ((kc + G - 1) / G > 8) G *= 2;
+        }
+        const int kpl = next_pow2((kc + G - 1) / G);  // 1..8; 8 and G < 64 only under `group`
         // Row order needs the dependency distance in flight, not a few levels: its default is
         // the hardware limit with 256- or 1024-thread workgroups.
         int bs = block > 0 ? next_pow2(std::min(std::max(block, 64), 1024))
@@ -486,7 +492,23 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
             else run_pull<T, 1, 1, 4, LEVELED>(bs, n, lds, s, a);
         } else if (G == 2) run_pull<T, 2, 1, 2, LEVELED>(bs, n, lds, s, a);
         else if (G == 4) run_pull<T, 4, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (G == 8) run_pull<T, 8, 1, 2, LEVELED>(bs, n, lds, s, a);
+        else if (G < 64 && kpl > 1) {
+            if constexpr (LEVELED) {
+                if (G == 8) {
+                    if (kpl == 2) run_pull<T, 8, 2, 2, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 8, 4, 1, true>(bs, n, lds, s, a);
+                    else run_pull<T, 8, 8, 1, true>(bs, n, lds, s, a);
+                } else if (G == 16) {
+                    if (kpl == 2) run_pull<T, 16, 2, 2, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 16, 4, 1, true>(bs, n, lds, s, a);
+                    else run_pull<T, 16, 8, 1, true>(bs, n, lds, s, a);
+                } else {
+                    if (kpl == 2) run_pull<T, 32, 2, 2, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 32, 4, 1, true>(bs, n, lds, s, a);
+                    else run_pull<T, 32, 8, 1, true>(bs, n, lds, s, a);
+                }
+            }
+        } else if (G == 8) run_pull<T, 8, 1, 2, LEVELED>(bs, n, lds, s, a);
         else if (G == 16) run_pull<T, 16, 1, 2, LEVELED>(bs, n, lds, s, a);
         else if (G == 32) run_pull<T, 32, 1, 2, LEVELED>(bs, n, lds, s, a);
         else if (kpl == 1) run_pull<T, 64, 1, 2, LEVELED>(bs, n, lds, s, a);

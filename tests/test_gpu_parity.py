@@ -190,6 +190,21 @@ def test_pull_launch_shapes(plans, block, occupancy):
                 assert np.array_equal(x, ref), (name, m, variant, O.rel_error(ref, x))
 
 
+@pytest.mark.parametrize("group", [8, 16, 32])
+def test_level_pull_lanes_per_row(plans, group):
+    """With several right-hand sides, `group` sets the lanes per row of the level-ordered pull
+    kernel (each lane then takes up to 8 columns): only the schedule changes, x stays bitwise
+    equal to the serial oracle, for widths that leave a partial last lane block too."""
+    for name in ("zipf_3000", "ic0_12", "chain_4000"):
+        U, P = MATS[name], plans[name]
+        for m in (5, 32, 100, 128, 300):
+            banner(f"lanes per row {name} m={m} group={group}")
+            xt, b = gen.rhs_for(U, m, seed=23, layout="row")
+            ref = oracle_solve(U, b)
+            x = P.solve_host(b, capi.params(variant=capi.VAR_LEVEL_PULL, group=group))
+            assert np.array_equal(x, ref), (name, m, group, O.rel_error(ref, x))
+
+
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
     m == 1 and everything else to the level-ordered pull kernel (bitwise exact)."""

@@ -350,6 +350,38 @@ def main():
                 extras[f"{wl}/{vname}/m{mm}"] = {"error": str(exc)}
             progress(f"extra {wl}/{vname}/m{mm}: {extras[f'{wl}/{vname}/m{mm}']}")
 
+    if not args.no_extras and m == 1:
+        # forward substitution with U^T (companion plan) and the full Cholesky solve U^T U x = b
+        # on the same plan and kernel family, against the oracle's triangular solves
+        cp, ri, cv = O.transpose(U.rowptr, U.col, U.val)
+        y_ref = O.base_gauss_lower(cp, ri, cv, b.reshape(-1))[0]
+        refs = {"transposed": y_ref, "cholesky": O.base_gauss_upper(U.rowptr, U.col, U.val, y_ref)[0]}
+        for kind, fn in (("transposed", P.solve_transposed), ("cholesky", P.solve_cholesky)):
+            key = f"{args.workload}/{args.variant}/m1/{kind}"
+            try:
+                d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+                for _ in range(2):
+                    fn(d_b, d_x, 1, prm)
+                P.sync()
+                reps = 10
+                P.timing(True)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn(d_b, d_x, 1, prm)
+                P.sync()
+                wall = (time.perf_counter() - t0) / reps
+                tot_ms, launches = P.timing_read()
+                P.timing(False)
+                xk = d_x.download().reshape(-1)
+                extras[key] = {"ms_per_step": round(wall * 1e3, 4), "kernel_ms": round(tot_ms / reps, 4),
+                               "solve_launches": launches // reps,
+                               "rel_err_vs_oracle": O.rel_error(refs[kind], xk)}
+                d_b.free()
+                d_x.free()
+            except Exception as exc:  # report, never hide
+                extras[key] = {"error": str(exc)}
+            progress(f"extra {key}: {extras[key]}")
+
     if not args.no_extras:
         # the host-buffer boundary (b uploaded, x downloaded inside the call): PCIe-inclusive rate
         P.solve_host(b, prm)

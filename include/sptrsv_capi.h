@@ -150,6 +150,21 @@ int sptrsv_solve_host(sptrsv_plan_t plan, const sptrsv_params_t* params, const d
 int sptrsv_plan_timing(sptrsv_plan_t plan, int enable);
 int sptrsv_plan_timing_read(sptrsv_plan_t plan, double* total_ms, int* launches);
 
+/* Forward substitution with the transpose: solve U^T X = B (lower triangular) on the same
+ * plan, m right-hand sides, device row-major n x m, asynchronous on the plan's stream.  The
+ * reference's forward solve base_gauss_lower (src/core.cpp:4-16) takes L in CSR with the
+ * diagonal last; L = U^T is exactly that layout of U's CSC copy.  A companion plan of
+ * J U^T J (J reverses the index order, which makes it upper triangular) is built on the first
+ * call and kept; params apply to its solve (SUPERNODAL is refused: that layout is built for U).
+ * Sums run in descending column order of L, so x agrees with base_gauss_lower to rounding. */
+int sptrsv_solve_transposed(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
+                            double* d_x, int m);
+/* Full Cholesky solve U^T U X = B: the transposed solve, then the solve with U (params apply
+ * to both; SUPERNODAL only to the second).  No reference counterpart: the reference solves
+ * one triangle per call. */
+int sptrsv_solve_cholesky(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
+                          double* d_x, int m);
+
 /* Residual r = b - U x on the device, FP64, row-major n x m. */
 int sptrsv_residual(sptrsv_plan_t plan, const double* d_b, const double* d_x, double* d_r, int m);
 

@@ -102,6 +102,8 @@ def lib():
         "sptrsv_plan_timing": (_i, [_PLAN, _i]),
         "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
         "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
+        "sptrsv_solve_transposed": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
+        "sptrsv_solve_cholesky": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
         "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
         "sptrsv_batch_create": (_i, [_i, _vp, _i64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "sptrsv_batch_destroy": (_i, [_vp]),
@@ -353,6 +355,37 @@ class Plan:
         check(lib().sptrsv_solve_refined(self.handle, ctypes.byref(prm or params()), ctypes.byref(o), d_b.ptr,
                                          d_x.ptr, int(m), ctypes.byref(it), ctypes.byref(ratio)), "solve_refined")
         return it.value, ratio.value
+
+    def solve_transposed(self, d_b: DeviceArray, d_x: DeviceArray, m: int = 1, prm: Params | None = None):
+        """Asynchronous device solve of U^T X = B (forward substitution, row-major n x m)."""
+        check(lib().sptrsv_solve_transposed(self.handle, ctypes.byref(prm or params()), d_b.ptr, d_x.ptr, int(m)),
+              "solve_transposed")
+
+    def solve_cholesky(self, d_b: DeviceArray, d_x: DeviceArray, m: int = 1, prm: Params | None = None):
+        """Asynchronous device solve of U^T U X = B (row-major n x m)."""
+        check(lib().sptrsv_solve_cholesky(self.handle, ctypes.byref(prm or params()), d_b.ptr, d_x.ptr, int(m)),
+              "solve_cholesky")
+
+    def _solve_dev_host(self, method, b, prm):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        m = 1 if b.ndim == 1 else b.shape[1]
+        d_b = DeviceArray.from_host(b)
+        d_x = DeviceArray(b.shape)
+        try:
+            method(d_b, d_x, m, prm)
+            self.sync()
+            return d_x.download()
+        finally:
+            d_b.free()
+            d_x.free()
+
+    def solve_transposed_host(self, b: np.ndarray, prm: Params | None = None):
+        """Synchronous U^T x = b for host b of shape (n,) or row-major (n, m)."""
+        return self._solve_dev_host(self.solve_transposed, b, prm)
+
+    def solve_cholesky_host(self, b: np.ndarray, prm: Params | None = None):
+        """Synchronous U^T U x = b for host b of shape (n,) or row-major (n, m)."""
+        return self._solve_dev_host(self.solve_cholesky, b, prm)
 
     def residual(self, d_b: DeviceArray, d_x: DeviceArray, d_r: DeviceArray, m: int = 1):
         check(lib().sptrsv_residual(self.handle, d_b.ptr, d_x.ptr, d_r.ptr, int(m)), "residual")

@@ -101,6 +101,14 @@ struct sptrsv_plan_s {
     int* tile_row = nullptr;      // row of each position
     double* tile_bt = nullptr;    // b in position order (per solve)
 
+    // companion plan of J U^T J (J reverses the index order) for solves with U^T, built on the
+    // first transposed solve and run on this plan's stream; tr_buf holds reversed b, reversed
+    // x and the intermediate of the Cholesky solve (n*m each)
+    sptrsv_plan_s* tr = nullptr;
+    double* tr_buf = nullptr;
+    size_t tr_elems = 0;
+    bool borrowed_stream = false;  // stream owned by the plan this one is the companion of
+
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -219,6 +219,8 @@ static int plan_alloc_common(sptrsv_plan_s* p) {
 int sptrsv_plan_destroy(sptrsv_plan_t p) {
     if (!p) return SPTRSV_OK;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->tr) sptrsv_plan_destroy(p->tr);
+    if (p->tr_buf) (void)hipFree(p->tr_buf);
     supernodal_free(p);
     tile_free(p);
     void* bufs[] = {p->rowptr, p->col, p->val, p->val32, p->colptr, p->rowidx, p->cval, p->cval32,
@@ -231,7 +233,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
-    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->stream && !p->borrowed_stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return SPTRSV_OK;
 }
@@ -339,6 +341,10 @@ int sptrsv_plan_timing(sptrsv_plan_t p, int enable) {
     SPT_HIP(hipStreamSynchronize(p->stream));
     p->timing = enable != 0;
     p->events_used = 0;
+    if (p->tr) {
+        p->tr->timing = p->timing;
+        p->tr->events_used = 0;
+    }
     return SPTRSV_OK;
 }
 
@@ -346,13 +352,18 @@ int sptrsv_plan_timing_read(sptrsv_plan_t p, double* total_ms, int* launches) {
     if (!p) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
     SPT_HIP(hipStreamSynchronize(p->stream));
     double tot = 0.0;
-    for (size_t i = 0; i < p->events_used; ++i) {
-        float ms = 0.f;
-        SPT_HIP(hipEventElapsedTime(&ms, p->events[i].first, p->events[i].second));
-        tot += ms;
+    int count = 0;
+    for (sptrsv_plan_s* q : {p, p->tr}) {  // the companion's launches are this plan's too
+        if (!q) continue;
+        for (size_t i = 0; i < q->events_used; ++i) {
+            float ms = 0.f;
+            SPT_HIP(hipEventElapsedTime(&ms, q->events[i].first, q->events[i].second));
+            tot += ms;
+        }
+        count += (int)q->events_used;
     }
     if (total_ms) *total_ms = tot;
-    if (launches) *launches = (int)p->events_used;
+    if (launches) *launches = count;
     return SPTRSV_OK;
 }
 
@@ -360,10 +371,17 @@ int sptrsv_plan_sync(sptrsv_plan_t p) {
     if (!p) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
     SPT_HIP(hipStreamSynchronize(p->stream));
     int status = 0;
-    SPT_HIP(hipMemcpy(&status, p->ticket + 1, sizeof(int), hipMemcpyDeviceToHost));
+    for (sptrsv_plan_s* q : {p, p->tr}) {  // the companion plan keeps its own status word
+        if (!q) continue;
+        int st = 0;
+        SPT_HIP(hipMemcpy(&st, q->ticket + 1, sizeof(int), hipMemcpyDeviceToHost));
+        if (st != 0) {
+            const int zero = 0;  // re-arm for the next solve
+            SPT_HIP(hipMemcpy(q->ticket + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
+            status = st;
+        }
+    }
     if (status != 0) {
-        const int zero = 0;  // re-arm for the next solve
-        SPT_HIP(hipMemcpy(p->ticket + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
         return fail(SPTRSV_ERR_DEADLOCK,
                     "device watchdog fired: a dependency was never published (cyclic or "
                     "malformed input, or the wait limit is too small)");

@@ -393,6 +393,65 @@ inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
     return (int)g;
 }
 
+// out = J in for row-major n x m (J reverses the row order)
+__global__ void k_reverse_rows(const double* __restrict__ in, double* __restrict__ out, int64_t n, int m) {
+    const int64_t count = n * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / m, k = e - i * m;
+        out[(n - 1 - i) * m + k] = in[e];
+    }
+}
+
+// U^T x = b is J (J U^T J) J x = b, and J U^T J is upper triangular: its row i' holds column
+// n-1-i' of U, entries reversed, so the diagonal (last in a column of the CSC copy) comes first
+// and the columns n-1-r ascend.  The companion plan is built once from the CSC copy and runs
+// on this plan's stream.
+int ensure_transposed(sptrsv_plan_s* P) {
+    if (P->tr) return SPTRSV_OK;
+    const int n = P->n;
+    const int64_t nnz = P->nnz;
+    std::vector<int> cp((size_t)n + 1), ri((size_t)nnz);
+    std::vector<double> cv((size_t)nnz);
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    SPT_HIP(hipMemcpy(cp.data(), P->colptr, cp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(ri.data(), P->rowidx, ri.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(cv.data(), P->cval, cv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int64_t> rp((size_t)n + 1, 0);
+    std::vector<int> c((size_t)nnz);
+    std::vector<double> v((size_t)nnz);
+    int64_t k = 0;
+    for (int ip = 0; ip < n; ++ip) {
+        const int j = n - 1 - ip;
+        for (int e = cp[j + 1] - 1; e >= cp[j]; --e) {
+            c[(size_t)k] = n - 1 - ri[(size_t)e];
+            v[(size_t)k] = cv[(size_t)e];
+            ++k;
+        }
+        rp[(size_t)ip + 1] = k;
+    }
+    sptrsv_plan_t t = nullptr;
+    SPT_CHECK(sptrsv_plan_create_csr(n, nnz, rp.data(), c.data(), v.data(), &t));
+    (void)hipStreamSynchronize(t->stream);
+    (void)hipStreamDestroy(t->stream);
+    t->stream = P->stream;
+    t->borrowed_stream = true;
+    t->timing = P->timing;
+    P->tr = t;
+    return SPTRSV_OK;
+}
+
+int ensure_tr_buf(sptrsv_plan_s* P, int m) {
+    const size_t need = 3 * (size_t)P->n * (size_t)m;
+    if (P->tr_elems >= need) return SPTRSV_OK;
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    if (P->tr_buf) SPT_HIP(hipFree(P->tr_buf));
+    P->tr_buf = nullptr;
+    P->tr_elems = 0;
+    SPT_HIP(hipMalloc((void**)&P->tr_buf, need * sizeof(double)));
+    P->tr_elems = need;
+    return SPTRSV_OK;
+}
+
 // Dynamic LDS that lets at most `occ` workgroups of a pull kernel reside on one CU (160 KiB of
 // LDS per CU; occ <= 0 = no cap).  Fewer resident rows far ahead of the dependency front means
 // fewer lanes polling x values that cannot be ready yet: every poll of an unpublished value is
@@ -803,6 +862,37 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
         return residual32(P, b, x, r, m);
     };
     return refine_loop(P, ro, d_b, d_x, m, solve_low, residual, iters, ratio);
+}
+
+int sptrsv_solve_transposed(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d_b, double* d_x,
+                            int m) {
+    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    if (params && params->variant == SPTRSV_VAR_SUPERNODAL)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "transposed solve: the supernodal layout is built for U only");
+    SPT_CHECK(ensure_transposed(P));
+    SPT_CHECK(ensure_tr_buf(P, m));
+    const int64_t n = P->n, nm = n * m;
+    double* rb = P->tr_buf;
+    double* rx = P->tr_buf + nm;
+    k_reverse_rows<<<grid_for(nm), 256, 0, P->stream>>>(d_b, rb, n, m);
+    SPT_HIP(hipGetLastError());
+    SPT_CHECK(sptrsv_solve(P->tr, params, rb, rx, m));
+    k_reverse_rows<<<grid_for(nm), 256, 0, P->stream>>>(rx, d_x, n, m);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int sptrsv_solve_cholesky(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d_b, double* d_x, int m) {
+    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    SPT_CHECK(ensure_tr_buf(P, m));
+    double* y = P->tr_buf + 2 * (int64_t)P->n * m;
+    sptrsv_params_t fwd{};
+    if (params) fwd = *params;
+    if (fwd.variant == SPTRSV_VAR_SUPERNODAL) fwd.variant = SPTRSV_VAR_AUTO;  // U^T has no supernodal layout
+    SPT_CHECK(sptrsv_solve_transposed(P, &fwd, d_b, y, m));
+    return sptrsv_solve(P, params, y, d_x, m);
 }
 
 int sptrsv_residual(sptrsv_plan_t P, const double* d_b, const double* d_x, double* d_r, int m) {

@@ -205,6 +205,66 @@ def test_level_pull_lanes_per_row(plans, group):
             assert np.array_equal(x, ref), (name, m, group, O.rel_error(ref, x))
 
 
+def lower_of(U):
+    """L = U^T as CSR with the diagonal last in each row (the CSC arrays of U)."""
+    return O.transpose(U.rowptr, U.col, U.val)
+
+
+def oracle_lower(U, b_rowmajor):
+    """the oracle's forward solve (src/core.cpp:4-16) with L = U^T, column by column"""
+    cp, ri, cv = lower_of(U)
+    if b_rowmajor.ndim == 1:
+        return O.base_gauss_lower(cp, ri, cv, b_rowmajor)[0]
+    return np.stack([O.base_gauss_lower(cp, ri, cv, np.ascontiguousarray(b_rowmajor[:, k]))[0]
+                     for k in range(b_rowmajor.shape[1])], axis=1)
+
+
+@pytest.mark.parametrize("variant", [capi.VAR_AUTO, capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET,
+                                     capi.VAR_CSC_PUSH, capi.VAR_CHAIN, capi.VAR_TILE])
+def test_transposed_solve(plans, variant):
+    """U^T x = b through the companion plan of J U^T J matches the oracle's forward solve of
+    L = U^T (diagonal last) to 1e-12, for every kernel family."""
+    for name in MATS:
+        U, P = MATS[name], plans[name]
+        for m in ((1,) if variant in (capi.VAR_CHAIN, capi.VAR_TILE) else (1, 3)):
+            banner(f"transposed {name} m={m} {capi.VARIANT_NAMES.get(variant, 'auto')}")
+            xt, b = gen.rhs_for(U, m, seed=31, layout="row")
+            if m == 1:
+                b = b.reshape(-1)
+            ref = oracle_lower(U, b)
+            x = P.solve_transposed_host(b, capi.params(variant=variant))
+            assert O.rel_error(ref, x) <= FP64_TOL, (name, m, O.rel_error(ref, x))
+
+
+def test_cholesky_solve(plans):
+    """U^T U x = b: the transposed solve, then the solve with U, against the oracle's two
+    triangular solves (1e-12), and the same x from every combination of launches."""
+    for name in MATS:
+        U, P = MATS[name], plans[name]
+        for m in (1, 4):
+            banner(f"cholesky {name} m={m}")
+            xt, b = gen.rhs_for(U, m, seed=32, layout="row")
+            if m == 1:
+                b = b.reshape(-1)
+            y = oracle_lower(U, b)
+            ref = oracle_solve(U, y)
+            x = P.solve_cholesky_host(b)
+            assert O.rel_error(ref, x) <= FP64_TOL, (name, m, O.rel_error(ref, x))
+
+
+def test_transposed_refuses_supernodal_and_times(plans):
+    P = plans["snode_10"]
+    U = MATS["snode_10"]
+    b = np.ones(U.n)
+    with pytest.raises(capi.SptrsvError):
+        P.solve_transposed_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+    P.timing(True)
+    P.solve_transposed_host(b)
+    ms, launches = P.timing_read()
+    P.timing(False)
+    assert launches >= 1 and ms > 0.0
+
+
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
     m == 1 and everything else to the level-ordered pull kernel (bitwise exact)."""

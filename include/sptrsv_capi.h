@@ -165,6 +165,14 @@ int sptrsv_solve_transposed(sptrsv_plan_t plan, const sptrsv_params_t* params, c
 int sptrsv_solve_cholesky(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
                           double* d_x, int m);
 
+/* The reference's loaders (src/utils.cpp:9-111) through the C ABI: .matrix (binary CSR),
+ * .mtx (Matrix Market lower factor, read as its upper transpose) and .txt matrices, .snodes and
+ * .txt supernode files.  Arrays are allocated with malloc; release them with sptrsv_free_host. */
+int sptrsv_read_matrix_file(const char* path, int* n, int64_t* nnz, int64_t** rowptr, int** col,
+                            double** val);
+int sptrsv_read_snodes_file(const char* path, int* sn, int** snodes /* sn + 1 */);
+void sptrsv_free_host(void* p);
+
 /* Residual r = b - U x on the device, FP64, row-major n x m. */
 int sptrsv_residual(sptrsv_plan_t plan, const double* d_b, const double* d_x, double* d_r, int m);
 

@@ -103,6 +103,10 @@ def lib():
         "sptrsv_plan_timing_read": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
         "sptrsv_residual": (_i, [_PLAN, _vp, _vp, _vp, _i]),
         "sptrsv_solve_transposed": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
+        "sptrsv_read_matrix_file": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i64),
+                                         ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+        "sptrsv_read_snodes_file": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_vp)]),
+        "sptrsv_free_host": (None, [_vp]),
         "sptrsv_solve_cholesky": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
         "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
         "sptrsv_batch_create": (_i, [_i, _vp, _i64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
@@ -166,6 +170,41 @@ def validate_csr(n, rowptr, col, val) -> int:
     if len(rp) != n + 1 or len(v) != len(c):
         return ERR_BAD_INPUT
     return lib().sptrsv_validate_csr(int(n), len(c), _ptr(rp), _ptr(c), _ptr(v))
+
+
+def _take_host(ptr, count, dtype):
+    """copy a library-allocated host array into numpy and release it"""
+    if not ptr.value:
+        return np.zeros(0, dtype=dtype)
+    buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr.value)
+    out = np.frombuffer(bytes(buf), dtype=dtype).copy()
+    lib().sptrsv_free_host(ptr)
+    return out
+
+
+def read_matrix(path: str):
+    """(n, rowptr int64, col int32, val f64) from a .matrix / .mtx / .txt file (the reference's
+    formats, read by the library's loader)."""
+    n, nnz = _i(0), _i64(0)
+    rp, c, v = _vp(), _vp(), _vp()
+    rc = lib().sptrsv_read_matrix_file(os.fsencode(path), ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(rp),
+                                       ctypes.byref(c), ctypes.byref(v))
+    if rc != OK:
+        for q in (rp, c, v):
+            if q.value:
+                lib().sptrsv_free_host(q)
+        raise SptrsvError(rc, "read_matrix", f"cannot read {path}")
+    return (n.value, _take_host(rp, n.value + 1, np.int64), _take_host(c, nnz.value, np.int32),
+            _take_host(v, nnz.value, np.float64))
+
+
+def read_snodes(path: str):
+    """supernode boundaries (sn + 1 ints) from a .snodes / .txt file"""
+    sn, p = _i(0), _vp()
+    rc = lib().sptrsv_read_snodes_file(os.fsencode(path), ctypes.byref(sn), ctypes.byref(p))
+    if rc != OK:
+        raise SptrsvError(rc, "read_snodes", f"cannot read {path}")
+    return _take_host(p, sn.value + 1, np.int32)
 
 
 def validate_csc(n, colptr, rowidx, val) -> int:

@@ -464,3 +464,35 @@ double sptrsv_syncfree3_opencl(int* csrColIdx, int* csrRowPtr, const VALUE_TYPE*
     return gpu_upper_csr("sptrsv_syncfree3_opencl", n, rp.data(), csrColIdx, csrVal, x, b, 1, 0,
                          SPTRSV_VAR_ROW_PULL);
 }
+
+// ---- C-ABI access to the loaders (for FFI callers; arrays are malloc'ed, free with
+// sptrsv_free_host) ----------------------------------------------------------------------------
+extern "C" int sptrsv_read_matrix_file(const char* path, int* n, int64_t* nnz, int64_t** rowptr, int** col,
+                                       double** val) {
+    if (!path || !n || !nnz || !rowptr || !col || !val) return SPTRSV_ERR_BAD_INPUT;
+    unsigned long long nz = 0;
+    uint64_t* row = nullptr;
+    *rowptr = nullptr;
+    *col = nullptr;
+    *val = nullptr;
+    if (read_csr(path, n, &nz, &row, col, val) < 0) {
+        free(row);
+        free(*col);
+        free(*val);
+        *col = nullptr;
+        *val = nullptr;
+        return SPTRSV_ERR_BAD_INPUT;
+    }
+    static_assert(sizeof(uint64_t) == sizeof(int64_t), "row pointer width");
+    *nnz = (int64_t)nz;
+    *rowptr = reinterpret_cast<int64_t*>(row);
+    return SPTRSV_OK;
+}
+
+extern "C" int sptrsv_read_snodes_file(const char* path, int* sn, int** snodes) {
+    if (!path || !sn || !snodes) return SPTRSV_ERR_BAD_INPUT;
+    read_snodes(path, sn, snodes);
+    return *snodes ? SPTRSV_OK : SPTRSV_ERR_BAD_INPUT;
+}
+
+extern "C" void sptrsv_free_host(void* p) { free(p); }

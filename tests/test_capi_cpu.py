@@ -162,3 +162,56 @@ def test_cpp_loaders_binary_and_mtx(tmp_path):
     assert "read failed" in bad.stdout and "Unknown extension" in bad.stderr
     missing = subprocess.run([exe, str(tmp_path / "none.matrix"), str(spath)], capture_output=True, text=True)
     assert "read failed" in missing.stdout
+
+
+def _write_matrix(path, U):
+    """.matrix binary layout (src/utils.cpp:17-31): int n, uint64 nz, int32 col, uint64 rowptr, f64 val"""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iQ", U.n, U.nnz))
+        f.write(np.asarray(U.col, dtype=np.int32).tobytes())
+        f.write(np.asarray(U.rowptr, dtype=np.uint64).tobytes())
+        f.write(np.asarray(U.val, dtype=np.float64).tobytes())
+
+
+def test_c_abi_loaders(tmp_path):
+    """sptrsv_read_matrix_file / sptrsv_read_snodes_file give the reference fixtures' arrays and a
+    .matrix / .snodes pair written here back bit for bit; a missing file is an error."""
+    n, rp, col, val = capi.read_matrix(os.path.join(GOLDEN, "csr_6_upper.txt"))
+    assert n == 6 and rp.tolist() == [0, 2, 6, 9, 10, 12, 13]
+    assert col.tolist() == [0, 3, 1, 2, 4, 5, 2, 4, 5, 3, 4, 5, 5]
+    assert val.tolist() == [1, 1, 1, 2, -1, 1, 1, 2, 3, 2, 4, 1, 6]
+    assert capi.read_snodes(os.path.join(GOLDEN, "csr_6_upper_snodes.txt")).tolist() == [0, 1, 3, 4, 6]
+    U = gen.random_supernodal(6, seed=3)
+    _write_matrix(tmp_path / "s.matrix", U)
+    with open(tmp_path / "s.snodes", "wb") as f:
+        f.write(struct.pack("<i", len(U.snodes) - 1))
+        f.write(np.asarray(U.snodes, dtype=np.int32).tobytes())
+    n, rp, col, val = capi.read_matrix(str(tmp_path / "s.matrix"))
+    assert n == U.n and np.array_equal(rp, U.rowptr) and np.array_equal(col, U.col) and np.array_equal(val, U.val)
+    assert np.array_equal(capi.read_snodes(str(tmp_path / "s.snodes")), U.snodes)
+    with pytest.raises(capi.SptrsvError):
+        capi.read_matrix(str(tmp_path / "missing.matrix"))
+
+
+def test_tables_layout(tmp_path, monkeypatch):
+    """times.csv keeps the reference driver's layout (run.py:34-58): ';' after every cell, a header
+    row of algorithms, one row per matrix file (supernode files skipped), min over the repeats."""
+    from sptrsv_amd import tables
+    for name in ("b.matrix", "a.mtx", "a.snodes", "c_snodes.txt", "c.txt", "notes.md"):
+        (tmp_path / name).write_text("")
+    times = iter([3.0, 1.0, 2.0, 5.0, 4.0, 6.0, 0.5, 0.7, 0.9, 8.0, 7.0, 9.0, 2.0, 2.0, 2.0, 1.0, 3.0, 1.0])
+    calls = []
+
+    def fake(algo, matrix, snodes, rhs):
+        calls.append((algo, os.path.basename(matrix), os.path.basename(snodes)))
+        return f"log {algo}\n", next(times)
+
+    monkeypatch.setattr(tables, "single_launch", fake)
+    monkeypatch.setattr("sys.argv", ["tables"])
+    tables.main([str(tmp_path), "--algos", "base,barrier", "--repeats", "3", "--out", str(tmp_path / "exp")])
+    (run,) = os.listdir(tmp_path / "exp")
+    rows = open(tmp_path / "exp" / run / "times.csv").read().splitlines()
+    assert rows[0] == ";base;barrier;"
+    assert rows[1:] == ["a.mtx;1.0;4.0;", "b.matrix;0.5;7.0;", "c.txt;2.0;1.0;"]
+    assert ("base", "a.mtx", "a.snodes") in calls and ("barrier", "c.txt", "c_snodes.txt") in calls
+    assert os.path.exists(tmp_path / "exp" / run / "best_times_log.log")

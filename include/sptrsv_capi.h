@@ -128,6 +128,15 @@ int sptrsv_plan_build_supernodes(sptrsv_plan_t plan, int sn, const int* snodes);
 /* Number of supernodes in use and (optionally, sn+1 entries) their boundaries. */
 int sptrsv_plan_get_supernodes(sptrsv_plan_t plan, int* sn, int* snodes);
 
+/* The reference's padded supernodal layout (ccs2ccs_pad, src/core.cpp:136-161) built on the
+ * device from the plan's CSR: row d of every supernode gets d zeros at the supernode's leading
+ * columns, so each supernode is a dense trapezoid.  snodes: sn + 1 host ints from 0 to n.
+ * Outputs are device arrays: d_val_pad, d_col_pad (uint64, the reference's type) of *nnz_pad
+ * entries, d_rowptr_pad n + 1 ints; *nnz_pad = nnz + sum_k w_k (w_k - 1) / 2.  With all three
+ * outputs null, only *nnz_pad is computed.  Synchronous. */
+int sptrsv_ccs2ccs_pad_device(sptrsv_plan_t plan, int sn, const int* snodes, double* d_val_pad,
+                              uint64_t* d_col_pad, int* d_rowptr_pad, int64_t* nnz_pad);
+
 /* Analysis results (device-computed, copied to host). */
 int sptrsv_plan_get_indegree(sptrsv_plan_t plan, int* indeg /* n */);
 int sptrsv_plan_get_levels(sptrsv_plan_t plan, int* levels /* n, 1-based */);

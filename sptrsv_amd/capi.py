@@ -107,6 +107,7 @@ def lib():
                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
         "sptrsv_read_snodes_file": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_vp)]),
         "sptrsv_free_host": (None, [_vp]),
+        "sptrsv_ccs2ccs_pad_device": (_i, [_PLAN, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
         "sptrsv_solve_cholesky": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
         "sptrsv_plan_build_supernodes": (_i, [_PLAN, _i, _vp]),
         "sptrsv_batch_create": (_i, [_i, _vp, _i64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
@@ -425,6 +426,22 @@ class Plan:
     def solve_cholesky_host(self, b: np.ndarray, prm: Params | None = None):
         """Synchronous U^T U x = b for host b of shape (n,) or row-major (n, m)."""
         return self._solve_dev_host(self.solve_cholesky, b, prm)
+
+    def ccs2ccs_pad(self, snodes):
+        """The padded supernodal layout built on the device: (val_pad, col_pad uint64, rowptr_pad)."""
+        s = np.ascontiguousarray(snodes, dtype=np.int32)
+        total = _i64(0)
+        check(lib().sptrsv_ccs2ccs_pad_device(self.handle, len(s) - 1, _ptr(s), None, None, None, ctypes.byref(total)),
+              "ccs2ccs_pad")
+        dv, dc = DeviceArray((total.value,)), DeviceArray((total.value,), np.uint64)
+        dr = DeviceArray((self.n + 1,), np.int32)
+        try:
+            check(lib().sptrsv_ccs2ccs_pad_device(self.handle, len(s) - 1, _ptr(s), dv.ptr, dc.ptr, dr.ptr,
+                                                  ctypes.byref(total)), "ccs2ccs_pad")
+            return dv.download(), dc.download(), dr.download()
+        finally:
+            for d in (dv, dc, dr):
+                d.free()
 
     def residual(self, d_b: DeviceArray, d_x: DeviceArray, d_r: DeviceArray, m: int = 1):
         check(lib().sptrsv_residual(self.handle, d_b.ptr, d_x.ptr, d_r.ptr, int(m)), "residual")

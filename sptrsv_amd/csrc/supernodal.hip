@@ -814,3 +814,81 @@ extern "C" int sptrsv_plan_get_supernodes(sptrsv_plan_t P, int* sn, int* snodes)
         for (size_t i = 0; i < P->h_snodes.size(); ++i) snodes[i] = P->h_snodes[i];
     return SPTRSV_OK;
 }
+
+namespace sptrsv {
+namespace {
+
+// Row r of supernode k (rows s_k .. s_{k+1}-1, d = r - s_k) gets d zeros at the supernode's
+// leading columns s_k .. s_k + d - 1, then its own entries: every supernode becomes a dense
+// trapezoid of width w_k + r_k.  Its padded start is rowptr[r] + pad[k] + d (d - 1) / 2, where
+// pad[k] counts the zeros of the earlier supernodes.
+__global__ void k_pad_rows(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+                           const double* __restrict__ val, int sn, const int* __restrict__ snodes,
+                           const int64_t* __restrict__ pad, double* __restrict__ val_pad,
+                           uint64_t* __restrict__ col_pad, int* __restrict__ rowptr_pad) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        int lo = 0, hi = sn;  // supernode of r: snodes[k] <= r < snodes[k + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (snodes[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        const int s = snodes[lo];
+        const int64_t d = r - s;
+        const int64_t start = rowptr[r] + pad[lo] + d * (d - 1) / 2;
+        for (int64_t z = 0; z < d; ++z) {
+            val_pad[start + z] = 0.0;
+            col_pad[start + z] = (uint64_t)(s + z);
+        }
+        const int p0 = rowptr[r], p1 = rowptr[r + 1];
+        for (int p = p0; p < p1; ++p) {
+            val_pad[start + d + (p - p0)] = val[p];
+            col_pad[start + d + (p - p0)] = (uint64_t)col[p];
+        }
+        rowptr_pad[r + 1] = (int)(start + d + (p1 - p0));
+        if (r == 0) rowptr_pad[0] = 0;
+    }
+}
+
+}  // namespace
+}  // namespace sptrsv
+
+extern "C" int sptrsv_ccs2ccs_pad_device(sptrsv_plan_t P, int sn, const int* snodes, double* d_val_pad,
+                                         uint64_t* d_col_pad, int* d_rowptr_pad, int64_t* nnz_pad) {
+    if (!P || !snodes || sn < 1) return fail(SPTRSV_ERR_BAD_INPUT, "null plan or empty partition");
+    if (snodes[0] != 0 || snodes[sn] != P->n)
+        return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must run from 0 to n");
+    std::vector<int64_t> pad((size_t)sn + 1, 0);
+    for (int k = 0; k < sn; ++k) {
+        const int64_t w = (int64_t)snodes[k + 1] - snodes[k];
+        if (w < 1) return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must increase");
+        pad[(size_t)k + 1] = pad[(size_t)k] + w * (w - 1) / 2;
+    }
+    const int64_t total = P->nnz + pad[(size_t)sn];
+    if (total >= ((int64_t)1 << 31)) return fail(SPTRSV_ERR_UNSUPPORTED, "padded nnz >= 2^31");
+    if (nnz_pad) *nnz_pad = total;
+    if (!d_val_pad && !d_col_pad && !d_rowptr_pad) return SPTRSV_OK;  // size query
+    if (!d_val_pad || !d_col_pad || !d_rowptr_pad) return fail(SPTRSV_ERR_BAD_INPUT, "null output");
+    int* d_sn = nullptr;
+    int64_t* d_pad = nullptr;
+    SPT_HIP(hipMalloc((void**)&d_sn, ((size_t)sn + 1) * sizeof(int)));
+    if (hipMalloc((void**)&d_pad, ((size_t)sn + 1) * sizeof(int64_t)) != hipSuccess) {
+        (void)hipFree(d_sn);
+        return fail(SPTRSV_ERR_NOMEM, "device allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(d_sn, snodes, ((size_t)sn + 1) * sizeof(int), hipMemcpyHostToDevice, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_pad, pad.data(), ((size_t)sn + 1) * sizeof(int64_t), hipMemcpyHostToDevice, P->stream);
+    if (e == hipSuccess) {
+        const int threads = 256;
+        const int blocks = (int)std::min<int64_t>((P->n + threads - 1) / threads, 65536);
+        k_pad_rows<<<blocks, threads, 0, P->stream>>>(P->n, P->rowptr, P->col, P->val, sn, d_sn, d_pad, d_val_pad,
+                                                      d_col_pad, d_rowptr_pad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(P->stream);  // the temporaries are freed next
+    (void)hipFree(d_sn);
+    (void)hipFree(d_pad);
+    if (e != hipSuccess) return fail(SPTRSV_ERR_HIP, std::string("ccs2ccs_pad_device: ") + hipGetErrorString(e));
+    return SPTRSV_OK;
+}

@@ -265,6 +265,38 @@ def test_transposed_refuses_supernodal_and_times(plans):
     assert launches >= 1 and ms > 0.0
 
 
+@pytest.mark.parametrize("name", ["csr_6", "snode_10", "sn_random", "one_row_snodes"])
+def test_ccs2ccs_pad_device(name):
+    """The device-built padded supernodal layout equals the oracle's ccs2ccs_pad (src/core.cpp:
+    136-161) entry for entry; the size query agrees."""
+    if name == "csr_6":
+        n, rp, col, val = capi.read_matrix(os.path.join(GOLDEN, "csr_6_upper.txt"))
+        U = gen.UpperCSR(n, rp, col, val)
+        snodes = capi.read_snodes(os.path.join(GOLDEN, "csr_6_upper_snodes.txt"))
+    elif name == "snode_10":
+        U = MATS["snode_10"]
+        snodes = U.snodes
+    elif name == "sn_random":
+        U = gen.random_supernodal(40, seed=9)
+        snodes = U.snodes
+    else:  # every row its own supernode: no padding
+        U = MATS["zipf_3000"]
+        snodes = np.arange(U.n + 1)
+    banner(f"ccs2ccs_pad {name}")
+    vp, cpad, rpp = O.ccs2ccs_pad(U.rowptr, U.col, U.val, snodes)
+    with capi.Plan.from_upper(U) as P:
+        gv, gc, gr = P.ccs2ccs_pad(snodes)
+    assert np.array_equal(gr, rpp) and np.array_equal(gc, cpad) and np.array_equal(gv.view(np.uint64), vp.view(np.uint64))
+
+
+def test_ccs2ccs_pad_device_rejects_bad_partition(plans):
+    P = plans["snode_10"]
+    with pytest.raises(capi.SptrsvError):
+        P.ccs2ccs_pad(np.array([0, 5]))  # does not reach n
+    with pytest.raises(capi.SptrsvError):
+        P.ccs2ccs_pad(np.array([0, 0, MATS["snode_10"].n]))  # empty supernode
+
+
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
     m == 1 and everything else to the level-ordered pull kernel (bitwise exact)."""

@@ -549,30 +549,30 @@ int launch_pull(sptrsv_plan_s* P, const PullArgs<T>& base, int m, int group, int
             if (unroll == 1) run_pull<T, 1, 1, 1, LEVELED>(bs, n, lds, s, a);
             else if (unroll == 2) run_pull<T, 1, 1, 2, LEVELED>(bs, n, lds, s, a);
             else run_pull<T, 1, 1, 4, LEVELED>(bs, n, lds, s, a);
-        } else if (G == 2) run_pull<T, 2, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (G == 4) run_pull<T, 4, 1, 2, LEVELED>(bs, n, lds, s, a);
+        } else if (G == 2) run_pull<T, 2, 1, 4, LEVELED>(bs, n, lds, s, a);
+        else if (G == 4) run_pull<T, 4, 1, 4, LEVELED>(bs, n, lds, s, a);
         else if (G < 64 && kpl > 1) {
             if constexpr (LEVELED) {
                 if (G == 8) {
-                    if (kpl == 2) run_pull<T, 8, 2, 2, true>(bs, n, lds, s, a);
-                    else if (kpl == 4) run_pull<T, 8, 4, 1, true>(bs, n, lds, s, a);
-                    else run_pull<T, 8, 8, 1, true>(bs, n, lds, s, a);
+                    if (kpl == 2) run_pull<T, 8, 2, 4, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 8, 4, 4, true>(bs, n, lds, s, a);
+                    else run_pull<T, 8, 8, 4, true>(bs, n, lds, s, a);
                 } else if (G == 16) {
-                    if (kpl == 2) run_pull<T, 16, 2, 2, true>(bs, n, lds, s, a);
-                    else if (kpl == 4) run_pull<T, 16, 4, 1, true>(bs, n, lds, s, a);
-                    else run_pull<T, 16, 8, 1, true>(bs, n, lds, s, a);
+                    if (kpl == 2) run_pull<T, 16, 2, 4, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 16, 4, 4, true>(bs, n, lds, s, a);
+                    else run_pull<T, 16, 8, 4, true>(bs, n, lds, s, a);
                 } else {
-                    if (kpl == 2) run_pull<T, 32, 2, 2, true>(bs, n, lds, s, a);
-                    else if (kpl == 4) run_pull<T, 32, 4, 1, true>(bs, n, lds, s, a);
-                    else run_pull<T, 32, 8, 1, true>(bs, n, lds, s, a);
+                    if (kpl == 2) run_pull<T, 32, 2, 4, true>(bs, n, lds, s, a);
+                    else if (kpl == 4) run_pull<T, 32, 4, 4, true>(bs, n, lds, s, a);
+                    else run_pull<T, 32, 8, 4, true>(bs, n, lds, s, a);
                 }
             }
-        } else if (G == 8) run_pull<T, 8, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (G == 16) run_pull<T, 16, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (G == 32) run_pull<T, 32, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (kpl == 1) run_pull<T, 64, 1, 2, LEVELED>(bs, n, lds, s, a);
-        else if (kpl == 2) run_pull<T, 64, 2, 1, LEVELED>(bs, n, lds, s, a);
-        else run_pull<T, 64, 4, 1, LEVELED>(bs, n, lds, s, a);
+        } else if (G == 8) run_pull<T, 8, 1, 4, LEVELED>(bs, n, lds, s, a);
+        else if (G == 16) run_pull<T, 16, 1, 4, LEVELED>(bs, n, lds, s, a);
+        else if (G == 32) run_pull<T, 32, 1, 4, LEVELED>(bs, n, lds, s, a);
+        else if (kpl == 1) run_pull<T, 64, 1, 4, LEVELED>(bs, n, lds, s, a);
+        else if (kpl == 2) run_pull<T, 64, 2, 4, LEVELED>(bs, n, lds, s, a);
+        else run_pull<T, 64, 4, 4, LEVELED>(bs, n, lds, s, a);
         end_timed(P, stop);
         SPT_HIP(hipGetLastError());
     }

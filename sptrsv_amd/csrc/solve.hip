@@ -103,7 +103,7 @@ __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
     wait_vmem();
     Watchdog wd;
     wd.arm(&a.ctrl[1], a.limit);
-    int idle = 0;
+    int idle = 0, streak = 0;
     while (true) {
         bool progressed = false;
         if (!done) {
@@ -150,8 +150,14 @@ __global__ __launch_bounds__(BS) void k_pull(PullArgs<T> a) {
             }
         }
         if (__all(done)) break;
+        // Back off while nothing moves: a wave far ahead of the dependency front polls less,
+        // which leaves the fabric to the polls at the front (ic0_128 m=1: 0.70 -> 0.675 ms).
+        if (__any(progressed)) streak = 0;
         if (!__any(progressed)) {
-            __builtin_amdgcn_s_sleep(1);
+            ++streak;
+            if (streak < 2) __builtin_amdgcn_s_sleep(1);
+            else if (streak < 8) __builtin_amdgcn_s_sleep(4);
+            else __builtin_amdgcn_s_sleep(12);
             if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
         }
     }

@@ -75,6 +75,15 @@ struct SnArgs {
 };
 
 constexpr int kSnCChunk = 1024;    // rectangular column indices staged in LDS at a time
+
+// Workgroup barrier that waits for this wave's LDS operations only (__syncthreads would also
+// drain the global loads issued ahead).
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt not waited
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
 constexpr int kSnLdt = kMaxW + 1;  // LDS row stride (doubles) of a 16-row block of Tinv
 
 // One workgroup of NW waves per (supernode k, panel of NW 16-column blocks of X), tickets from
@@ -153,41 +162,89 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve(SnAr
     }
 
     // ---- W = B_s - A X(cols) over K segments (solve: R_s; residual: T_s, then R_s) --------
-    double rreg[kPerR];
+    // One wave (NW == 1) has registers to spare at its occupancy: it keeps TWO chunks of R and
+    // X in flight (sets A and B) and waits for a chunk one iteration after issuing it.
+    constexpr bool kDeep = NW == 1;
+    double rreg[kPerR], rreg2[kDeep ? kPerR : 1];
     constexpr int KS = kSnKC / 4;  // MFMA K-steps per chunk
     double bv[KS], bn[KS];
     // one K segment: A is w x K row-major (leading dimension lda); column kk of the segment is
     // X row sC[kk] (cols != nullptr, staged in LDS by blocks) or row s0 + kk (cols == nullptr)
     auto segment = [&](const double* A, int lda, int K, const int* cols) {
-        auto load_r = [&](int kg) {  // A[:, kg .. kg+KC) into registers (zeros outside)
+        auto load_r_to = [&](double* rr, int kg) {  // A[:, kg .. kg+KC) into registers (zeros outside)
 #pragma unroll
             for (int q = 0; q < kPerR; ++q) {
                 const int e = tid + q * NT, row = e / kSnKC, kc = kg + e % kSnKC;
-                rreg[q] = (row < w && kc < K) ? A[(size_t)row * lda + kc] : 0.0;
+                rr[q] = (row < w && kc < K) ? A[(size_t)row * lda + kc] : 0.0;
             }
         };
-        auto store_r = [&](double* dst) {
+        // A goes to LDS negated (the store waits for its loads anyway), so acc = B + (-A) X needs
+        // no negation of the X loads, which would make each of them wait as it lands
+        auto store_r_from = [&](const double* rr, double* dst) {
 #pragma unroll
             for (int q = 0; q < kPerR; ++q) {
                 const int e = tid + q * NT;
-                if (e / kSnKC < kMaxW) dst[(e / kSnKC) * kSnLdr + e % kSnKC] = rreg[q];
+                if (e / kSnKC < kMaxW) dst[(e / kSnKC) * kSnLdr + e % kSnKC] = -rr[q];
             }
         };
+        auto load_r = [&](int kg) { load_r_to(rreg, kg); };
+        auto store_r = [&](double* dst) { store_r_from(rreg, dst); };
         for (int cb = 0; cb < K; cb += kSnCChunk) {
             const int cn = min(kSnCChunk, K - cb);
             const int nch = (cn + kSnKC - 1) / kSnKC;
-            auto load_x = [&](double* out, int kl) {  // negated X rows of local chunk kl
+            auto load_x = [&](double* out, int kl) {  // X rows of local chunk kl
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
                     const int kk = kl + 4 * s + quad;
                     const int row = cols ? sC[kk < cn ? kk : 0] : s0 + cb + kk;
-                    out[s] = (kk < cn && jv) ? -a.x[(size_t)row * m + jj] : 0.0;
+                    out[s] = (kk < cn && jv) ? a.x[(size_t)row * m + jj] : 0.0;
                 }
             };
             __syncthreads();
             if (cols)
                 for (int q = tid; q < cn; q += NT) sC[q] = cols[cb + q];
             __syncthreads();
+            if constexpr (kDeep) {
+                // chunk c: R in LDS buffer c & 1, X in set (c & 1 ? bn : bv) ... rotated below:
+                // at the top of iteration c, chunk c is in LDS with its X in xa, chunk c + 1 is
+                // in flight in rb / xb; the iteration multiplies c, stores c + 1 to LDS, and
+                // issues c + 2 into the set c used.
+                double xa[KS], xb[KS];
+                load_x(xa, 0);
+                load_r_to(rreg, cb);
+                if (nch > 1) {
+                    load_x(xb, kSnKC);
+                    load_r_to(rreg2, cb + kSnKC);
+                }
+                store_r_from(rreg, sbuf);
+                lds_barrier();
+                auto step = [&](int c, double (&xc)[KS], double (&xn)[KS], double* rc, const double* rn) {
+                    const double* cur = sbuf + (c & 1) * (kMaxW * kSnLdr);
+                    double* nxt = sbuf + ((c + 1) & 1) * (kMaxW * kSnLdr);
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+                        for (int I = 0; I < 8; ++I) {
+                            if (I < nI) {
+                                const double av = cur[(16 * I + c16) * kSnLdr + 4 * s + quad];
+                                acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xc[s], acc[I], 0, 0, 0);
+                            }
+                        }
+                    }
+                    if (c + 1 < nch) store_r_from(rn, nxt);
+                    if (c + 2 < nch) {
+                        load_x(xc, kSnKC * (c + 2));
+                        load_r_to(rc, cb + kSnKC * (c + 2));
+                    }
+                    lds_barrier();
+                    (void)xn;
+                };
+                for (int c = 0; c < nch; c += 2) {
+                    step(c, xa, xb, rreg, rreg2);
+                    if (c + 1 < nch) step(c + 1, xb, xa, rreg2, rreg);
+                }
+                continue;
+            }
             load_x(bn, 0);
             load_r(cb);
             store_r(sbuf);

@@ -259,6 +259,7 @@ def main():
     ap.add_argument("--workload", default="ic0_128")
     ap.add_argument("--variant", default="level_pull")
     ap.add_argument("--m", type=int, default=1)
+    ap.add_argument("--occupancy", type=int, default=0, help="params.occupancy of the measured solve")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -288,7 +289,7 @@ def main():
     xt, b = gen.rhs_for(U, m, seed=1, layout="row")
     d_b = capi.DeviceArray.from_host(b)
     d_x = capi.DeviceArray(b.shape)
-    prm = capi.params(variant=variant)
+    prm = capi.params(variant=variant, occupancy=args.occupancy)
     first = []
     ms, kms = time_solves(capi, P, d_b, d_x, m, prm, args.steps, args.warmup, first)
     x = d_x.download()

@@ -55,7 +55,13 @@ extern "C" {
                                      by level with a barrier between levels, dependencies inside the
                                      tile read from LDS, later tiles polled; m == 1, FP64, exact
                                      reference summation order */
-#define SPTRSV_VAR_COUNT 8
+#define SPTRSV_VAR_BLOCK 8        /* one wave per block of <= 64 rows, blocks in a topological
+                                     order (4x4x4 / 8x8 bricks of a natural-order grid factor,
+                                     else bands of 16 levels), dependencies inside a block through
+                                     LDS, on other blocks polled; params.occupancy = resident
+                                     blocks per CU (0 = 2); m == 1, FP64, exact reference
+                                     summation order */
+#define SPTRSV_VAR_COUNT 9
 
 /* Value precision of the solve. */
 #define SPTRSV_FP64 0
@@ -140,6 +146,10 @@ int sptrsv_ccs2ccs_pad_device(sptrsv_plan_t plan, int sn, const int* snodes, dou
 /* Analysis results (device-computed, copied to host). */
 int sptrsv_plan_get_indegree(sptrsv_plan_t plan, int* indeg /* n */);
 int sptrsv_plan_get_levels(sptrsv_plan_t plan, int* levels /* n, 1-based */);
+/* Block order of the BLOCK variant (no reference counterpart), built here if no block solve
+ * built it yet: *n_blocks blocks, *kind 1 = bricks of a natural-order grid factor, 2 = level
+ * bands; block_ptr (n_blocks + 1 positions) and perm (position -> row, n) may be null. */
+int sptrsv_plan_get_blocks(sptrsv_plan_t plan, int* n_blocks, int* kind, int* block_ptr, int* perm);
 int sptrsv_plan_get_level_sets(sptrsv_plan_t plan, int* level_ptr /* n_levels+1 */,
                                int* level_rows /* n */);
 int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /* nnz */,

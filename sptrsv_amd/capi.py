@@ -30,9 +30,10 @@ VAR_LEVELSET = 4
 VAR_CHAIN = 5
 VAR_SUPERNODAL = 6
 VAR_TILE = 7
+VAR_BLOCK = 8
 VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
                  VAR_LEVELSET: "levelset", VAR_CHAIN: "chain",
-                 VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile"}
+                 VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile", VAR_BLOCK: "block"}
 
 FP64 = 0
 FP32 = 1
@@ -94,6 +95,7 @@ def lib():
         "sptrsv_plan_info": (_i, [_PLAN, ctypes.POINTER(_i), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
         "sptrsv_plan_get_indegree": (_i, [_PLAN, _vp]),
         "sptrsv_plan_get_levels": (_i, [_PLAN, _vp]),
+        "sptrsv_plan_get_blocks": (_i, [_PLAN, _vp, _vp, _vp, _vp]),
         "sptrsv_plan_get_level_sets": (_i, [_PLAN, _vp, _vp]),
         "sptrsv_plan_get_csc": (_i, [_PLAN, _vp, _vp, _vp]),
         "sptrsv_solve": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
@@ -336,6 +338,17 @@ class Plan:
         out = np.zeros(self.n, dtype=np.int32)
         check(lib().sptrsv_plan_get_levels(self.handle, _ptr(out)), "get_levels")
         return out
+
+    def blocks(self):
+        """block order of the BLOCK variant: (kind, block_ptr, perm); kind 1 = grid bricks,
+        2 = level bands"""
+        nb = np.zeros(1, dtype=np.int32)
+        kind = np.zeros(1, dtype=np.int32)
+        check(lib().sptrsv_plan_get_blocks(self.handle, _ptr(nb), _ptr(kind), None, None), "get_blocks")
+        ptr = np.zeros(int(nb[0]) + 1, dtype=np.int32)
+        perm = np.zeros(self.n, dtype=np.int32)
+        check(lib().sptrsv_plan_get_blocks(self.handle, _ptr(nb), _ptr(kind), _ptr(ptr), _ptr(perm)), "get_blocks")
+        return int(kind[0]), ptr, perm
 
     def level_sets(self):
         ptr = np.zeros(self.n_levels + 1, dtype=np.int32)

@@ -1,0 +1,433 @@
+// block.hip -- block pull (m == 1, FP64): one workgroup per block of up to 64 rows, the
+// block's internal dependencies handed over through LDS.
+//
+// In the row and level pull kernels every dependency is a device-scope store-then-poll round
+// trip (1-2 us), so the solve costs about one round trip per level of the dependency DAG.
+// Here the rows are cut into blocks of at most 64 rows, the blocks are taken in a topological
+// order (every dependency of a block lies in the same block or in an earlier one), and one
+// workgroup of two waves solves one block:
+//   * the compute wave holds a row per lane and reads every value it needs from LDS: the
+//     block's own rows from their slots as soon as their lanes store them, the values of
+//     earlier blocks from slots the poller fills; it never waits on global memory;
+//   * the poller wave holds the block's list of external values (at most kMaxExt) and polls
+//     the self-publishing x for them until all have arrived, copying each into its LDS slot.
+// A dependency inside the block costs an LDS round trip; only a dependency on another block
+// costs a device round trip, and the compute wave's progress never waits on the poller's.
+//
+// Blocks: for a factor whose off-diagonal offsets are those of a natural-order 2D or 3D grid
+// ({1, nx} or {1, nx, nx*ny}), 8x8 or 4x4x4 bricks ordered by brick anti-diagonal (a path
+// through a 128^3 grid crosses ~93 bricks instead of 381 levels); otherwise (or if the brick
+// order is not topological) bands of 16 levels with rows in descending order inside a band,
+// cut into chunks of 64.  A block is cut short where its external list would exceed kMaxExt.
+//
+// Each lane sums its row's products in ascending column order and divides by the diagonal,
+// the reference serial solver's arithmetic (src/core.cpp:18-32), so x is bitwise equal to it.
+// Workgroups hold tickets in block order and wait only on earlier blocks, whose workgroups
+// are already resident, which guarantees progress; every wait is bounded by the watchdog.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace sptrsv {
+namespace {
+
+constexpr int kBlockRows = 64;
+constexpr int kExtPerLane = 8;
+constexpr int kMaxExt = kWave * kExtPerLane;  // external values per block
+constexpr int kSlots = kBlockRows + kMaxExt;  // LDS slots: the block's rows, then its externals
+constexpr int kBand = 16;                      // levels per band of the generic block order
+
+struct BlockArgs {
+    const int* __restrict__ bptr;   // nblk + 1 block boundaries (positions)
+    const int* __restrict__ eptr;   // nblk + 1 offsets into ecol
+    const int* __restrict__ ecol;   // external values of each block (rows of earlier blocks)
+    const int* __restrict__ bperm;  // position -> row
+    const int* __restrict__ brp;    // CSR in position order (diagonal first)
+    const int* __restrict__ bslot;  // LDS slot of each entry's x value
+    const double* __restrict__ bval;
+    const double* __restrict__ b;
+    double* x;
+    int* ctrl;
+    uint64_t limit;
+};
+
+__device__ __forceinline__ uint64_t lds_get(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int UNROLL>
+__global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
+#pragma clang fp contract(off)
+    using U = uint64_t;
+    constexpr U S = Bits<double>::sentinel;
+    __shared__ U slot[kSlots];
+    __shared__ int s_ticket;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool poller = threadIdx.x >= kWave;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
+    if (!poller) lds_put(&slot[lane], S);
+#pragma unroll
+    for (int j = 0; j < kExtPerLane; ++j)
+        if (poller) lds_put(&slot[kBlockRows + lane + kWave * j], S);
+    __syncthreads();
+    const int t = s_ticket;
+    Watchdog wd;
+    wd.arm(&a.ctrl[1], a.limit);
+    int idle = 0, streak = 0;
+    if (poller) {
+        const int e0 = a.eptr[t], ne = a.eptr[t + 1] - e0;
+        int ec[kExtPerLane];
+        bool need[kExtPerLane];
+        U v[kExtPerLane];
+#pragma unroll
+        for (int j = 0; j < kExtPerLane; ++j) {
+            const int k = lane + kWave * j;
+            need[j] = k < ne;
+            ec[j] = a.ecol[e0 + k];  // ecol is padded by kMaxExt entries: always in bounds
+            v[j] = S;
+        }
+        wait_vmem();
+        while (true) {
+#pragma unroll
+            for (int j = 0; j < kExtPerLane; ++j)
+                if (need[j]) v[j] = poll_bits(a.x + ec[j]);
+            bool got = false, left = false;
+#pragma unroll
+            for (int j = 0; j < kExtPerLane; ++j) {
+                if (need[j] && v[j] != S) {
+                    lds_put(&slot[kBlockRows + lane + kWave * j], v[j]);
+                    need[j] = false;
+                    got = true;
+                }
+                left |= need[j];
+            }
+            if (!__any(left)) break;
+            if (__any(got)) streak = 0;
+            if (!__any(got)) {
+                ++streak;
+                if (streak < 2) __builtin_amdgcn_s_sleep(1);
+                else if (streak < 8) __builtin_amdgcn_s_sleep(4);
+                else __builtin_amdgcn_s_sleep(12);
+                if ((++idle & 63) == 0 && wd.expired()) break;
+            }
+        }
+        return;
+    }
+    const int p0 = a.bptr[t], p1 = a.bptr[t + 1];
+    const int pos = p0 + lane;
+    bool done = pos >= p1;
+    int row = 0, p = 0, end = 0, cnt = 0, q = 0;
+    double diag = 1.0, acc = 0.0, bv = 0.0;
+    int sl[UNROLL];
+    double v[UNROLL];
+    U xb[UNROLL];
+    auto load_window = [&]() {
+        cnt = min(UNROLL, end - p);
+        q = 0;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            sl[u] = 0;
+            v[u] = 0.0;
+            if (u < cnt) {
+                sl[u] = a.bslot[p + u];
+                v[u] = a.bval[p + u];
+            }
+            xb[u] = S;
+        }
+    };
+    if (!done) {
+        row = a.bperm[pos];
+        const int s = a.brp[pos];
+        diag = a.bval[s];
+        p = s + 1;
+        end = a.brp[pos + 1];
+        bv = a.b[row];
+        load_window();
+    }
+    wait_vmem();
+    while (true) {
+        bool prog = false;
+        if (!done) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (u >= q && u < cnt && xb[u] == S) xb[u] = lds_get(&slot[sl[u]]);
+            bool stop = false;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                if (u >= q && u < cnt && !stop) {
+                    if (xb[u] != S) {
+                        acc = acc + v[u] * __builtin_bit_cast(double, xb[u]);
+                        q = u + 1;
+                        prog = true;
+                    } else {
+                        stop = true;
+                    }
+                }
+            }
+            if (q == cnt) {
+                if (p + cnt < end) {
+                    p += cnt;
+                    load_window();
+                    wait_vmem();
+                } else {
+                    const double xi = (bv - acc) / diag;
+                    lds_put(&slot[lane], __builtin_bit_cast(U, xi));
+                    publish(a.x + row, xi);
+                    done = true;
+                }
+                prog = true;
+            }
+        }
+        if (__all(done)) break;
+        if (!__any(prog)) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 255) == 0 && !done && wd.expired()) done = true;
+        }
+    }
+}
+
+__global__ void k_fill_sentinel_b(double* x, int64_t n) {
+    const uint64_t s = Bits<double>::sentinel;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        reinterpret_cast<uint64_t*>(x)[i] = s;
+}
+
+__global__ void k_reset_ticket_block(int* ctrl) {
+    if (threadIdx.x == 0) ctrl[0] = 0;
+}
+
+template <typename X>
+int upload_b(X** dst, const std::vector<X>& src) {
+    SPT_HIP(hipMalloc((void**)dst, std::max<size_t>(src.size(), 1) * sizeof(X)));
+    if (!src.empty()) SPT_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(X), hipMemcpyHostToDevice));
+    return SPTRSV_OK;
+}
+
+// Grid of a natural-order stencil factor: the distinct off-diagonal offsets are {1, nx} or
+// {1, nx, nx*ny} with n a multiple of the largest.  Returns false for anything else.
+bool detect_grid(int n, const std::vector<int>& rp, const std::vector<int>& col, int64_t* dims) {
+    std::map<int, int64_t> offs;
+    for (int r = 0; r < n; ++r)
+        for (int e = rp[r] + 1; e < rp[r + 1]; ++e) {
+            ++offs[col[e] - r];
+            if (offs.size() > 3) return false;
+        }
+    std::vector<int> o;
+    for (auto& kv : offs) o.push_back(kv.first);
+    if (o.empty() || o[0] != 1) return false;
+    if (o.size() == 2 && o[1] > 1 && n % o[1] == 0) {
+        dims[0] = o[1];
+        dims[1] = n / o[1];
+        dims[2] = 1;
+        return true;
+    }
+    if (o.size() == 3 && o[1] > 1 && o[2] > o[1] && o[2] % o[1] == 0 && n % o[2] == 0) {
+        dims[0] = o[1];
+        dims[1] = o[2] / o[1];
+        dims[2] = n / o[2];
+        return true;
+    }
+    return false;
+}
+
+// Block order (position -> row) and block boundaries.  `kind` reports which order was used:
+// 1 = grid bricks, 2 = level bands.
+void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col, const std::vector<int>& lev,
+                 std::vector<int>& order, std::vector<int>& bptr, int* kind) {
+    int64_t d[3];
+    order.assign((size_t)n, 0);
+    bptr.clear();
+    if (detect_grid(n, rp, col, d)) {
+        const int64_t bx = d[2] > 1 ? 4 : 8, by = d[2] > 1 ? 4 : 8, bz = d[2] > 1 ? 4 : 1;
+        const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
+        // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
+        // coordinates), then descending brick index; rows inside a brick descending
+        std::vector<int64_t> key((size_t)n);
+        for (int r = 0; r < n; ++r) {
+            const int64_t i = r % d[0], j = (r / d[0]) % d[1], k = r / (d[0] * d[1]);
+            const int64_t I = i / bx, J = j / by, K = k / bz;
+            const int64_t diagk = (gx - 1 - I) + (gy - 1 - J) + (gz - 1 - K);
+            const int64_t bidx = (gx * gy * gz - 1) - (I + gx * (J + gy * K));
+            key[(size_t)r] = diagk * (gx * gy * gz) + bidx;
+        }
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return key[(size_t)x] != key[(size_t)y] ? key[(size_t)x] < key[(size_t)y] : x > y;
+        });
+        for (int pos = 0; pos < n; ++pos)
+            if (pos == 0 || key[(size_t)order[(size_t)pos]] != key[(size_t)order[(size_t)pos - 1]] ||
+                pos - bptr.back() == kBlockRows)
+                bptr.push_back(pos);
+        bptr.push_back(n);
+        // topological check: every dependency in the same or an earlier block
+        std::vector<int> blk((size_t)n);
+        for (size_t b = 0; b + 1 < bptr.size(); ++b)
+            for (int pos = bptr[b]; pos < bptr[b + 1]; ++pos) blk[(size_t)order[(size_t)pos]] = (int)b;
+        bool ok = true;
+        for (int r = 0; r < n && ok; ++r)
+            for (int e = rp[r] + 1; e < rp[r + 1]; ++e)
+                if (blk[(size_t)col[e]] > blk[(size_t)r]) {
+                    ok = false;
+                    break;
+                }
+        if (ok) {
+            *kind = 1;
+            return;
+        }
+        bptr.clear();
+    }
+    // level bands: a dependency has a lower level, so it lies in an earlier band or, in the
+    // same band, at a larger row index -- an earlier position
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int bx = (lev[(size_t)x] - 1) / kBand, by = (lev[(size_t)y] - 1) / kBand;
+        return bx != by ? bx < by : x > y;
+    });
+    for (int pos = 0; pos < n; pos += kBlockRows) bptr.push_back(pos);
+    bptr.push_back(n);
+    *kind = 2;
+}
+
+int build_blocks(sptrsv_plan_s* P) {
+    const int n = P->n;
+    const int64_t nnz = P->nnz;
+    std::vector<int> rp((size_t)n + 1), col((size_t)nnz), lev((size_t)n);
+    std::vector<double> val((size_t)nnz);
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    SPT_HIP(hipMemcpy(rp.data(), P->rowptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(lev.data(), P->levels, lev.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r)
+        if (rp[r + 1] - rp[r] - 1 > kMaxExt)
+            return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: row " + std::to_string(r) + " has more than " +
+                                                    std::to_string(kMaxExt) + " off-diagonal entries");
+    std::vector<int> order, pre;
+    int kind = 0;
+    block_order(n, rp, col, lev, order, pre, &kind);
+    std::vector<int> posof((size_t)n);
+    for (int pos = 0; pos < n; ++pos) posof[(size_t)order[(size_t)pos]] = pos;
+    // cut blocks where the external list would exceed kMaxExt; external lists in order of
+    // first use.  A dependency sits at an earlier position, so it is either in the block
+    // (position >= the block's start) or external.
+    std::vector<int> bptr, eptr, ecol, stamp((size_t)n, -1), extidx((size_t)n, 0);
+    std::vector<int> brp((size_t)n + 1), bslot((size_t)nnz);
+    std::vector<double> bval((size_t)nnz);
+    int64_t e0 = 0;
+    int nb = -1, start = 0, next_pre = 0;
+    auto open_block = [&](int pos) {
+        ++nb;
+        start = pos;
+        bptr.push_back(pos);
+        eptr.push_back((int)ecol.size());
+    };
+    for (int pos = 0; pos < n; ++pos) {
+        const int r = order[(size_t)pos];
+        bool cut = pos == pre[(size_t)next_pre];
+        if (cut) ++next_pre;
+        if (!cut) {
+            int fresh = 0;
+            for (int e = rp[r] + 1; e < rp[r + 1]; ++e) {
+                const int c = col[e];
+                fresh += posof[(size_t)c] < start && stamp[(size_t)c] != nb;
+            }
+            cut = (int)ecol.size() - eptr.back() + fresh > kMaxExt;
+        }
+        if (cut) open_block(pos);
+        brp[(size_t)pos] = (int)e0;
+        bslot[(size_t)e0] = 0;
+        bval[(size_t)e0] = val[rp[r]];
+        ++e0;
+        for (int e = rp[r] + 1; e < rp[r + 1]; ++e, ++e0) {
+            const int c = col[e];
+            const int pc = posof[(size_t)c];
+            if (pc >= start) {
+                bslot[(size_t)e0] = pc - start;
+            } else {
+                if (stamp[(size_t)c] != nb) {
+                    stamp[(size_t)c] = nb;
+                    extidx[(size_t)c] = (int)ecol.size() - eptr.back();
+                    ecol.push_back(c);
+                }
+                bslot[(size_t)e0] = kBlockRows + extidx[(size_t)c];
+            }
+            bval[(size_t)e0] = val[e];
+        }
+    }
+    brp[(size_t)n] = (int)e0;
+    bptr.push_back(n);
+    eptr.push_back((int)ecol.size());
+    ecol.resize(ecol.size() + kMaxExt, 0);  // the poller loads kMaxExt entries from every block's start
+    block_free(P);
+    SPT_CHECK(upload_b(&P->blk_ptr, bptr));
+    SPT_CHECK(upload_b(&P->blk_eptr, eptr));
+    SPT_CHECK(upload_b(&P->blk_ecol, ecol));
+    SPT_CHECK(upload_b(&P->blk_perm, order));
+    SPT_CHECK(upload_b(&P->blk_rp, brp));
+    SPT_CHECK(upload_b(&P->blk_col, bslot));
+    SPT_CHECK(upload_b(&P->blk_val, bval));
+    P->n_blocks = (int)bptr.size() - 1;
+    P->blk_kind = kind;
+    return SPTRSV_OK;
+}
+
+}  // namespace
+
+void block_free(sptrsv_plan_s* p) {
+    void* bufs[] = {p->blk_ptr, p->blk_eptr, p->blk_ecol, p->blk_perm, p->blk_rp, p->blk_col, p->blk_val};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    p->blk_ptr = p->blk_eptr = p->blk_ecol = p->blk_perm = p->blk_rp = p->blk_col = nullptr;
+    p->blk_val = nullptr;
+    p->n_blocks = 0;
+    p->blk_kind = 0;
+}
+
+int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, double* d_x, int m, uint64_t limit) {
+    if (m != 1) return fail(SPTRSV_ERR_UNSUPPORTED, "block variant solves one right-hand side");
+    if (precision != SPTRSV_FP64) return fail(SPTRSV_ERR_UNSUPPORTED, "block variant is FP64");
+    if (P->n == 0) return SPTRSV_OK;
+    if (P->n_blocks == 0) SPT_CHECK(build_blocks(P));
+    const int64_t n = P->n;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n);
+    k_reset_ticket_block<<<1, 64, 0, P->stream>>>(P->ticket);
+    BlockArgs a{P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_val,
+                d_b, d_x, P->ticket, limit};
+    // resident blocks per CU: dynamic LDS sized so that at most `occ` workgroups share a CU
+    // (160 KiB of LDS per CU; the kernel's own slots take kSlots * 8 bytes)
+    const int o = occ > 0 ? std::min(occ, 16) : 2;
+    const size_t lds = o >= 16 ? 0 : ((size_t)(163840 / o - kSlots * 8 - 512) & ~(size_t)255);
+    hipEvent_t stop;
+    SPT_CHECK(begin_timed(P, &stop));
+    k_block<4><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
+    end_timed(P, stop);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" int sptrsv_plan_get_blocks(sptrsv_plan_t P, int* n_blocks, int* kind, int* block_ptr, int* perm) {
+    if (!P || !n_blocks || !kind) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (P->n > 0 && P->n_blocks == 0) SPT_CHECK(build_blocks(P));
+    *n_blocks = P->n_blocks;
+    *kind = P->blk_kind;
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    if (block_ptr && P->n_blocks > 0)
+        SPT_HIP(hipMemcpy(block_ptr, P->blk_ptr, ((size_t)P->n_blocks + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (perm && P->n > 0) SPT_HIP(hipMemcpy(perm, P->blk_perm, (size_t)P->n * sizeof(int), hipMemcpyDeviceToHost));
+    return SPTRSV_OK;
+}

@@ -101,6 +101,20 @@ struct sptrsv_plan_s {
     int* tile_row = nullptr;      // row of each position
     double* tile_bt = nullptr;    // b in position order (per solve)
 
+    // blocks of the BLOCK variant (built on first use): block k = positions [blk_ptr[k],
+    // blk_ptr[k+1]), position p = row blk_perm[p]; CSR in position order whose column entries
+    // (blk_col) are LDS slots: < 64 a row of the block, 64 + e its e-th external value;
+    // blk_kind 1 = grid bricks, 2 = level bands
+    int n_blocks = 0;
+    int blk_kind = 0;
+    int* blk_ptr = nullptr;
+    int* blk_eptr = nullptr;     // per block: its external values blk_ecol[blk_eptr[k] ..)
+    int* blk_ecol = nullptr;
+    int* blk_perm = nullptr;
+    int* blk_rp = nullptr;
+    int* blk_col = nullptr;
+    double* blk_val = nullptr;
+
     // companion plan of J U^T J (J reverses the index order) for solves with U^T, built on the
     // first transposed solve and run on this plan's stream; tr_buf holds reversed b, reversed
     // x and the intermediate of the Cholesky solve (n*m each)
@@ -135,6 +149,9 @@ int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, fl
 int tile_solve(sptrsv_plan_s* p, int precision, int rows, const double* d_b, double* d_x, int m,
                uint64_t limit);  // tile.hip
 void tile_free(sptrsv_plan_s* p);         // tile.hip
+int block_solve(sptrsv_plan_s* p, int precision, int occ, const double* d_b, double* d_x, int m,
+                uint64_t limit);  // block.hip
+void block_free(sptrsv_plan_s* p);        // block.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
 int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip

@@ -771,6 +771,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     const int unroll = prm.unroll > 0 ? prm.unroll : 4;
     if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, prm.block, d_b, d_x, m, limit);
+    if (variant == SPTRSV_VAR_BLOCK) return block_solve(P, prm.precision, prm.occupancy, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_CHAIN) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, prm.occupancy, prm.block, d_b, d_x, m, P->val, P->cval, P->lval, limit);
@@ -846,8 +847,8 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     if (opts) ro = *opts;
     const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
     int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
-    if (variant == SPTRSV_VAR_TILE)
-        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile variant has no reduced-precision form");
+    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile and block variants have no reduced-precision form");
     if (variant == SPTRSV_VAR_CHAIN && m != 1)
         return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the chain variant solves one right-hand side");
     if (variant == SPTRSV_VAR_SUPERNODAL ? low != SPTRSV_BF16 : low != SPTRSV_FP32)

@@ -1,0 +1,134 @@
+"""BLOCK variant: one wave per block of <= 64 rows with the block's internal dependencies passed
+through LDS.  Checks the block order the library builds (a partition into blocks of 1..64 rows
+whose every dependency lies in the same or an earlier block; 4x4x4 / 8x8 bricks for
+natural-order grid factors, level bands otherwise) on the CPU from the arrays it returns, and
+FP64 x bitwise equal to the serial oracle (the kernel keeps the reference's summation order)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sptrsv_amd import capi, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def offsets_factor(n, offsets, seed, keep=None):
+    """upper factor with entries (r, r + o) for o in offsets (kept where keep(r, o) is true),
+    diagonal in [2, 3], off-diagonals in [-0.3, 0.3]"""
+    rng = np.random.default_rng(seed)
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for o in offsets:
+        r = np.arange(n - o)
+        if keep is not None:
+            r = r[keep(r, o)]
+        rows.append(r)
+        cols.append(r + o)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    vals = np.where(rows == cols, rng.uniform(2, 3, len(rows)), rng.uniform(-0.3, 0.3, len(rows)))
+    o = np.lexsort((cols, rows))
+    return gen._from_rows(n, rows[o], cols[o], vals[o], "offsets")
+
+
+def grid2d(nx, ny, seed=1):
+    return offsets_factor(nx * ny, (1, nx), seed, keep=lambda r, o: (o != 1) | (r % nx != nx - 1))
+
+
+def grid3d(nx, ny, nz, seed=2):
+    def keep(r, o):
+        i, j = r % nx, (r // nx) % ny
+        return ~(((o == 1) & (i == nx - 1)) | ((o == nx) & (j == ny - 1)))
+    return offsets_factor(nx * ny * nz, (1, nx, nx * ny), seed, keep)
+
+
+CASES = {
+    # name: (factor, expected kind)
+    "ic0_12": (gen.laplace3d_ic0(12), 1),
+    "ic0_10": (gen.laplace3d_ic0(10), 1),        # partial bricks at the far faces
+    "grid3d_9x6x5": (grid3d(9, 6, 5), 1),
+    "grid2d_20x13": (grid2d(20, 13), 1),
+    "grid2d_8x8": (grid2d(8, 8), 1),             # one brick
+    # offsets {1, 16} with the +1 entries wrapping from line to line: the brick order is not
+    # topological, so the level-band order is used
+    "wrap_16x20": (offsets_factor(16 * 20, (1, 16), 3), 2),
+    "zipf_3000": (gen.random_zipf(3000, seed=11), 2),
+    "chol2d_15": (gen.laplace2d_cholesky(15), 2),
+    "chain_300": (offsets_factor(300, (1,), 4), 2),
+    "diag_only": (gen.UpperCSR(100, np.arange(101), np.arange(100, dtype=np.int32), np.linspace(1, 2, 100)), 2),
+}
+
+
+def check_order(U, kind, ptr, perm):
+    n = U.n
+    assert ptr[0] == 0 and ptr[-1] == n
+    sizes = np.diff(ptr)
+    assert sizes.min() >= 1 and sizes.max() <= 64
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    blk = np.empty(n, dtype=np.int64)
+    blk[perm] = np.repeat(np.arange(len(sizes)), sizes)
+    rp = np.asarray(U.rowptr)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = np.asarray(U.col) != rows
+    assert np.all(blk[np.asarray(U.col)[off]] <= blk[rows[off]]), "a dependency lies in a later block"
+    return blk
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_block_order_and_parity(name):
+    U, kind = CASES[name]
+    banner(f"block {name} n={U.n}")
+    with capi.Plan.from_upper(U) as P:
+        got_kind, ptr, perm = P.blocks()
+        assert got_kind == kind
+        check_order(U, got_kind, ptr, perm)
+        xt, b = gen.rhs_for(U, 1, seed=5)
+        ref = O.base_gauss_upper(U.rowptr, U.col, U.val, b)[0]
+        for occ in (0, 1, 4, 32):
+            x = P.solve_host(b, capi.params(variant=capi.VAR_BLOCK, occupancy=occ))
+            assert np.array_equal(x, ref), (name, occ, O.rel_error(ref, x))
+
+
+def test_bricks_of_ic0_12():
+    """12^3 grid: 27 bricks of 4x4x4 rows, each block one brick"""
+    U = CASES["ic0_12"][0]
+    with capi.Plan.from_upper(U) as P:
+        kind, ptr, perm = P.blocks()
+    assert kind == 1 and len(ptr) == 28 and np.all(np.diff(ptr) == 64)
+    N = 12
+    for b in range(27):
+        r = perm[ptr[b]:ptr[b + 1]]
+        cells = {(int(i) // 4, int(j) // 4, int(k) // 4) for i, j, k in zip(r % N, (r // N) % N, r // (N * N))}
+        assert len(cells) == 1
+    # bricks by descending anti-diagonal: the first block is the far corner brick
+    r = perm[:64]
+    assert np.all(r % N >= 8) and np.all((r // N) % N >= 8) and np.all(r // (N * N) >= 8)
+
+
+def test_block_rejects():
+    U = CASES["ic0_12"][0]
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 2, seed=1, layout="row")
+        with pytest.raises(capi.SptrsvError) as e:
+            P.solve_host(b, capi.params(variant=capi.VAR_BLOCK))
+        assert e.value.code == capi.ERR_UNSUPPORTED
+        xt, b = gen.rhs_for(U, 1, seed=1)
+        with pytest.raises(capi.SptrsvError) as e:
+            P.solve_host(b, capi.params(variant=capi.VAR_BLOCK, precision=capi.FP32))
+        assert e.value.code == capi.ERR_UNSUPPORTED
+
+
+def test_block_transposed():
+    """the transposed solve U^T y = b runs the same kernel on the companion plan of J U^T J
+    (again a grid factor, so again bricks)"""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve_triangular
+    U = CASES["grid3d_9x6x5"][0]
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 1, seed=9)
+        got = P.solve_transposed_host(b, capi.params(variant=capi.VAR_BLOCK))
+    A = sp.csr_matrix((U.val, U.col, U.rowptr), shape=(U.n, U.n))
+    ref = spsolve_triangular(A.T.tocsr(), b, lower=True)
+    assert O.rel_error(ref, got) <= 1e-12

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 FP32_TOL = 2e-5
 EXACT = [capi.VAR_ROW_PULL, capi.VAR_LEVEL_PULL, capi.VAR_LEVELSET]
-EXACT_M1 = EXACT + [capi.VAR_TILE]  # the tile kernel solves one right-hand side
+EXACT_M1 = EXACT + [capi.VAR_TILE, capi.VAR_BLOCK]  # the tile and block kernels solve one right-hand side
 
 
 def banner(msg):
@@ -220,13 +220,13 @@ def oracle_lower(U, b_rowmajor):
 
 
 @pytest.mark.parametrize("variant", [capi.VAR_AUTO, capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET,
-                                     capi.VAR_CSC_PUSH, capi.VAR_CHAIN, capi.VAR_TILE])
+                                     capi.VAR_CSC_PUSH, capi.VAR_CHAIN, capi.VAR_TILE, capi.VAR_BLOCK])
 def test_transposed_solve(plans, variant):
     """U^T x = b through the companion plan of J U^T J matches the oracle's forward solve of
     L = U^T (diagonal last) to 1e-12, for every kernel family."""
     for name in MATS:
         U, P = MATS[name], plans[name]
-        for m in ((1,) if variant in (capi.VAR_CHAIN, capi.VAR_TILE) else (1, 3)):
+        for m in ((1,) if variant in (capi.VAR_CHAIN, capi.VAR_TILE, capi.VAR_BLOCK) else (1, 3)):
             banner(f"transposed {name} m={m} {capi.VARIANT_NAMES.get(variant, 'auto')}")
             xt, b = gen.rhs_for(U, m, seed=31, layout="row")
             if m == 1:
@@ -373,7 +373,7 @@ def test_full_size_ic0_128():
         assert np.array_equal(P.indegree(), (1 + (x < N - 1) + (y < N - 1) + (z < N - 1)).astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=1)
         ref = oracle_solve(U, b)
-        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_TILE):
+        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_TILE, capi.VAR_BLOCK):
             banner(f"solve ic0_128 {capi.VARIANT_NAMES[variant]}")
             got = P.solve_host(b, capi.params(variant=variant))
             assert np.array_equal(got, ref)
@@ -391,7 +391,7 @@ def test_full_size_zipf_1e6_level_pull():
         assert np.array_equal(P.levels(), lv.astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=1)
         ref = oracle_solve(U, b)
-        for variant in (capi.VAR_ROW_PULL, capi.VAR_TILE):
+        for variant in (capi.VAR_ROW_PULL, capi.VAR_TILE, capi.VAR_BLOCK):
             banner(f"solve zipf_1e6 {capi.VARIANT_NAMES[variant]}")
             got = P.solve_host(b, capi.params(variant=variant))
             assert np.array_equal(got, ref)

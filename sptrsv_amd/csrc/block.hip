@@ -51,6 +51,7 @@ struct BlockArgs {
     const int* __restrict__ bperm;  // position -> row
     const int* __restrict__ brp;    // CSR in position order (diagonal first)
     const int* __restrict__ bslot;  // LDS slot of each entry's x value
+    const int* __restrict__ bstep;  // step of each position inside its block
     const double* __restrict__ bval;
     const double* __restrict__ b;
     double* x;
@@ -123,75 +124,84 @@ __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
         }
         return;
     }
+    // compute wave: rows in steps; a row's step is one more than the largest step among its
+    // dependencies inside the block (0 without any), so at step s every in-block value it
+    // reads was stored at an earlier step of this same wave; only its external values may
+    // still be missing, and the lanes of step s wait for those in LDS.
     const int p0 = a.bptr[t], p1 = a.bptr[t + 1];
     const int pos = p0 + lane;
-    bool done = pos >= p1;
-    int row = 0, p = 0, end = 0, cnt = 0, q = 0;
-    double diag = 1.0, acc = 0.0, bv = 0.0;
+    const bool valid = pos < p1;
+    int row = 0, step = -1, p = 0, end = 0;
+    double diag = 1.0, bv = 0.0;
     int sl[UNROLL];
     double v[UNROLL];
-    U xb[UNROLL];
-    auto load_window = [&]() {
-        cnt = min(UNROLL, end - p);
-        q = 0;
+    auto load_window = [&](int q) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             sl[u] = 0;
             v[u] = 0.0;
-            if (u < cnt) {
-                sl[u] = a.bslot[p + u];
-                v[u] = a.bval[p + u];
+            if (q + u < end) {
+                sl[u] = a.bslot[q + u];
+                v[u] = a.bval[q + u];
             }
-            xb[u] = S;
         }
     };
-    if (!done) {
+    if (valid) {
         row = a.bperm[pos];
+        step = a.bstep[pos];
         const int s = a.brp[pos];
         diag = a.bval[s];
         p = s + 1;
         end = a.brp[pos + 1];
         bv = a.b[row];
-        load_window();
+        load_window(p);
     }
     wait_vmem();
-    while (true) {
-        bool prog = false;
-        if (!done) {
+    int nsteps = step + 1;
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-                if (u >= q && u < cnt && xb[u] == S) xb[u] = lds_get(&slot[sl[u]]);
-            bool stop = false;
+    for (int o = 32; o >= 1; o >>= 1) nsteps = max(nsteps, __shfl_xor(nsteps, o));
+    bool dead = false;
+    for (int st = 0; st < nsteps && !dead; ++st) {
+        const bool mine = step == st;
+        double acc = 0.0;
+        for (int q = p;; q += UNROLL) {
+            const bool active = mine && q < end;
+            if (!__any(active)) break;
+            if (q != p) {  // rows longer than one window: next window from memory
+                if (active) load_window(q);
+                wait_vmem();
+            }
+            const int cnt = min(UNROLL, end - q);
+            U xv[UNROLL];
+            while (true) {
+                bool ok = true;
+                if (active) {
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                if (u >= q && u < cnt && !stop) {
-                    if (xb[u] != S) {
-                        acc = acc + v[u] * __builtin_bit_cast(double, xb[u]);
-                        q = u + 1;
-                        prog = true;
-                    } else {
-                        stop = true;
+                    for (int u = 0; u < UNROLL; ++u) {
+                        xv[u] = lds_get(&slot[sl[u]]);
+                        ok &= u >= cnt || xv[u] != S;
                     }
                 }
-            }
-            if (q == cnt) {
-                if (p + cnt < end) {
-                    p += cnt;
-                    load_window();
-                    wait_vmem();
-                } else {
-                    const double xi = (bv - acc) / diag;
-                    lds_put(&slot[lane], __builtin_bit_cast(U, xi));
-                    publish(a.x + row, xi);
-                    done = true;
+                if (__all(ok)) break;
+                __builtin_amdgcn_s_sleep(1);
+                if ((++idle & 255) == 0 && wd.expired()) {
+                    dead = true;
+                    break;
                 }
-                prog = true;
+            }
+            if (dead) break;
+            if (active) {
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const double t2 = acc + v[u] * __builtin_bit_cast(double, xv[u]);
+                    acc = u < cnt ? t2 : acc;
+                }
             }
         }
-        if (__all(done)) break;
-        if (!__any(prog)) {
-            __builtin_amdgcn_s_sleep(1);
-            if ((++idle & 255) == 0 && !done && wd.expired()) done = true;
+        if (mine && !dead) {
+            const double xi = (bv - acc) / diag;
+            lds_put(&slot[lane], __builtin_bit_cast(U, xi));
+            publish(a.x + row, xi);
         }
     }
 }
@@ -321,7 +331,7 @@ int build_blocks(sptrsv_plan_s* P) {
     // first use.  A dependency sits at an earlier position, so it is either in the block
     // (position >= the block's start) or external.
     std::vector<int> bptr, eptr, ecol, stamp((size_t)n, -1), extidx((size_t)n, 0);
-    std::vector<int> brp((size_t)n + 1), bslot((size_t)nnz);
+    std::vector<int> brp((size_t)n + 1), bslot((size_t)nnz), bstep((size_t)n, 0);
     std::vector<double> bval((size_t)nnz);
     int64_t e0 = 0;
     int nb = -1, start = 0, next_pre = 0;
@@ -353,6 +363,7 @@ int build_blocks(sptrsv_plan_s* P) {
             const int pc = posof[(size_t)c];
             if (pc >= start) {
                 bslot[(size_t)e0] = pc - start;
+                bstep[(size_t)pos] = std::max(bstep[(size_t)pos], bstep[(size_t)pc] + 1);
             } else {
                 if (stamp[(size_t)c] != nb) {
                     stamp[(size_t)c] = nb;
@@ -375,6 +386,7 @@ int build_blocks(sptrsv_plan_s* P) {
     SPT_CHECK(upload_b(&P->blk_perm, order));
     SPT_CHECK(upload_b(&P->blk_rp, brp));
     SPT_CHECK(upload_b(&P->blk_col, bslot));
+    SPT_CHECK(upload_b(&P->blk_step, bstep));
     SPT_CHECK(upload_b(&P->blk_val, bval));
     P->n_blocks = (int)bptr.size() - 1;
     P->blk_kind = kind;
@@ -384,10 +396,10 @@ int build_blocks(sptrsv_plan_s* P) {
 }  // namespace
 
 void block_free(sptrsv_plan_s* p) {
-    void* bufs[] = {p->blk_ptr, p->blk_eptr, p->blk_ecol, p->blk_perm, p->blk_rp, p->blk_col, p->blk_val};
+    void* bufs[] = {p->blk_ptr, p->blk_eptr, p->blk_ecol, p->blk_perm, p->blk_rp, p->blk_col, p->blk_step, p->blk_val};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    p->blk_ptr = p->blk_eptr = p->blk_ecol = p->blk_perm = p->blk_rp = p->blk_col = nullptr;
+    p->blk_ptr = p->blk_eptr = p->blk_ecol = p->blk_perm = p->blk_rp = p->blk_col = p->blk_step = nullptr;
     p->blk_val = nullptr;
     p->n_blocks = 0;
     p->blk_kind = 0;
@@ -402,8 +414,8 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
     k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n);
     k_reset_ticket_block<<<1, 64, 0, P->stream>>>(P->ticket);
-    BlockArgs a{P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_val,
-                d_b, d_x, P->ticket, limit};
+    BlockArgs a{P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
+                P->blk_val, d_b, d_x, P->ticket, limit};
     // resident blocks per CU: dynamic LDS sized so that at most `occ` workgroups share a CU
     // (160 KiB of LDS per CU; the kernel's own slots take kSlots * 8 bytes)
     const int o = occ > 0 ? std::min(occ, 16) : 2;

@@ -113,6 +113,7 @@ struct sptrsv_plan_s {
     int* blk_perm = nullptr;
     int* blk_rp = nullptr;
     int* blk_col = nullptr;
+    int* blk_step = nullptr;     // per position: 1 + the largest step of its in-block dependencies
     double* blk_val = nullptr;
 
     // companion plan of J U^T J (J reverses the index order) for solves with U^T, built on the

@@ -59,7 +59,7 @@ extern "C" {
                                      order (4x4x4 / 8x8 bricks of a natural-order grid factor,
                                      else bands of 16 levels), dependencies inside a block through
                                      LDS, on other blocks polled; params.occupancy = resident
-                                     blocks per CU (0 = 2); m == 1, FP64, exact reference
+                                     blocks per CU (0 = 3); m == 1, FP64, exact reference
                                      summation order */
 #define SPTRSV_VAR_COUNT 9
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int UNROLL>
+template <int UNROLL, bool SHORT>
 __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
 #pragma clang fp contract(off)
     using U = uint64_t;
@@ -112,6 +112,8 @@ __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
                 }
                 left |= need[j];
             }
+            // wake the compute wave, which sleeps while the values of its step are missing
+            if (__any(got)) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_wakeup" ::: "memory");
             if (!__any(left)) break;
             if (__any(got)) streak = 0;
             if (!__any(got)) {
@@ -164,18 +166,14 @@ __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
     for (int st = 0; st < nsteps && !dead; ++st) {
         const bool mine = step == st;
         double acc = 0.0;
-        for (int q = p;; q += UNROLL) {
-            const bool active = mine && q < end;
-            if (!__any(active)) break;
-            if (q != p) {  // rows longer than one window: next window from memory
-                if (active) load_window(q);
-                wait_vmem();
-            }
-            const int cnt = min(UNROLL, end - q);
+        if constexpr (SHORT) {
+            // every row fits one window: read the step's values, wait in LDS for missing
+            // external ones, then the ordered sum
+            const int cnt = end - p;
             U xv[UNROLL];
             while (true) {
                 bool ok = true;
-                if (active) {
+                if (mine) {
 #pragma unroll
                     for (int u = 0; u < UNROLL; ++u) {
                         xv[u] = lds_get(&slot[sl[u]]);
@@ -183,18 +181,52 @@ __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
                     }
                 }
                 if (__all(ok)) break;
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(10);
                 if ((++idle & 255) == 0 && wd.expired()) {
                     dead = true;
                     break;
                 }
             }
-            if (dead) break;
-            if (active) {
+            if (mine) {
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
                     const double t2 = acc + v[u] * __builtin_bit_cast(double, xv[u]);
                     acc = u < cnt ? t2 : acc;
+                }
+            }
+        } else {
+            for (int q = p;; q += UNROLL) {
+                const bool active = mine && q < end;
+                if (!__any(active)) break;
+                if (q != p) {  // rows longer than one window: next window from memory
+                    if (active) load_window(q);
+                    wait_vmem();
+                }
+                const int cnt = min(UNROLL, end - q);
+                U xv[UNROLL];
+                while (true) {
+                    bool ok = true;
+                    if (active) {
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) {
+                            xv[u] = lds_get(&slot[sl[u]]);
+                            ok &= u >= cnt || xv[u] != S;
+                        }
+                    }
+                    if (__all(ok)) break;
+                    __builtin_amdgcn_s_sleep(10);
+                    if ((++idle & 255) == 0 && wd.expired()) {
+                        dead = true;
+                        break;
+                    }
+                }
+                if (dead) break;
+                if (active) {
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) {
+                        const double t2 = acc + v[u] * __builtin_bit_cast(double, xv[u]);
+                        acc = u < cnt ? t2 : acc;
+                    }
                 }
             }
         }
@@ -390,6 +422,8 @@ int build_blocks(sptrsv_plan_s* P) {
     SPT_CHECK(upload_b(&P->blk_val, bval));
     P->n_blocks = (int)bptr.size() - 1;
     P->blk_kind = kind;
+    P->blk_maxlen = 0;
+    for (int r = 0; r < n; ++r) P->blk_maxlen = std::max(P->blk_maxlen, rp[r + 1] - rp[r] - 1);
     return SPTRSV_OK;
 }
 
@@ -403,6 +437,7 @@ void block_free(sptrsv_plan_s* p) {
     p->blk_val = nullptr;
     p->n_blocks = 0;
     p->blk_kind = 0;
+    p->blk_maxlen = 0;
 }
 
 int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, double* d_x, int m, uint64_t limit) {
@@ -418,11 +453,12 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
                 P->blk_val, d_b, d_x, P->ticket, limit};
     // resident blocks per CU: dynamic LDS sized so that at most `occ` workgroups share a CU
     // (160 KiB of LDS per CU; the kernel's own slots take kSlots * 8 bytes)
-    const int o = occ > 0 ? std::min(occ, 16) : 2;
+    const int o = occ > 0 ? std::min(occ, 16) : 3;
     const size_t lds = o >= 16 ? 0 : ((size_t)(163840 / o - kSlots * 8 - 512) & ~(size_t)255);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    k_block<4><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
+    if (P->blk_maxlen <= 4) k_block<4, true><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
+    else k_block<4, false><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

@@ -107,6 +107,7 @@ struct sptrsv_plan_s {
     // blk_kind 1 = grid bricks, 2 = level bands
     int n_blocks = 0;
     int blk_kind = 0;
+    int blk_maxlen = 0;          // most off-diagonal entries in a row
     int* blk_ptr = nullptr;
     int* blk_eptr = nullptr;     // per block: its external values blk_ecol[blk_eptr[k] ..)
     int* blk_ecol = nullptr;

@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ic0_128")
-    ap.add_argument("--variant", default="level_pull")
+    ap.add_argument("--variant", default="block")
     ap.add_argument("--m", type=int, default=1)
     ap.add_argument("--occupancy", type=int, default=0, help="params.occupancy of the measured solve")
     ap.add_argument("--cpu-budget", type=float, default=10.0)

@@ -155,6 +155,11 @@ int sptrsv_plan_get_level_sets(sptrsv_plan_t plan, int* level_ptr /* n_levels+1 
 int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /* nnz */,
                         double* val /* nnz */);
 
+/* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
+ * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average, BLOCK for FP64 m == 1
+ * on a natural-order grid factor, LEVEL_PULL otherwise. */
+int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
+
 /* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).
  * Asynchronous on the plan's stream. */
 int sptrsv_solve(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,

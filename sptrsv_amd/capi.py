@@ -96,6 +96,7 @@ def lib():
         "sptrsv_plan_get_indegree": (_i, [_PLAN, _vp]),
         "sptrsv_plan_get_levels": (_i, [_PLAN, _vp]),
         "sptrsv_plan_get_blocks": (_i, [_PLAN, _vp, _vp, _vp, _vp]),
+        "sptrsv_plan_auto_variant": (_i, [_PLAN, _i, _i, _vp]),
         "sptrsv_plan_get_level_sets": (_i, [_PLAN, _vp, _vp]),
         "sptrsv_plan_get_csc": (_i, [_PLAN, _vp, _vp, _vp]),
         "sptrsv_solve": (_i, [_PLAN, ctypes.POINTER(Params), _vp, _vp, _i]),
@@ -338,6 +339,12 @@ class Plan:
         out = np.zeros(self.n, dtype=np.int32)
         check(lib().sptrsv_plan_get_levels(self.handle, _ptr(out)), "get_levels")
         return out
+
+    def auto_variant(self, m=1, precision=FP64):
+        """the variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision`"""
+        out = np.zeros(1, dtype=np.int32)
+        check(lib().sptrsv_plan_auto_variant(self.handle, m, precision, _ptr(out)), "auto_variant")
+        return int(out[0])
 
     def blocks(self):
         """block order of the BLOCK variant: (kind, block_ptr, perm); kind 1 = grid bricks,

@@ -36,6 +36,34 @@
 #include "plan.h"
 
 namespace sptrsv {
+
+// Grid of a natural-order stencil factor: the distinct off-diagonal offsets are {1, nx} or
+// {1, nx, nx*ny} with n a multiple of the largest.  Returns false for anything else.
+bool detect_grid(int n, const int* rp, const int* col, int64_t* dims) {
+    std::map<int, int64_t> offs;
+    for (int r = 0; r < n; ++r)
+        for (int e = rp[r] + 1; e < rp[r + 1]; ++e) {
+            ++offs[col[e] - r];
+            if (offs.size() > 3) return false;
+        }
+    std::vector<int> o;
+    for (auto& kv : offs) o.push_back(kv.first);
+    if (o.empty() || o[0] != 1) return false;
+    if (o.size() == 2 && o[1] > 1 && n % o[1] == 0) {
+        dims[0] = o[1];
+        dims[1] = n / o[1];
+        dims[2] = 1;
+        return true;
+    }
+    if (o.size() == 3 && o[1] > 1 && o[2] > o[1] && o[2] % o[1] == 0 && n % o[2] == 0) {
+        dims[0] = o[1];
+        dims[1] = o[2] / o[1];
+        dims[2] = n / o[2];
+        return true;
+    }
+    return false;
+}
+
 namespace {
 
 constexpr int kBlockRows = 64;
@@ -50,7 +78,8 @@ struct BlockArgs {
     const int* __restrict__ ecol;   // external values of each block (rows of earlier blocks)
     const int* __restrict__ bperm;  // position -> row
     const int* __restrict__ brp;    // CSR in position order (diagonal first)
-    const int* __restrict__ bslot;  // LDS slot of each entry's x value
+    const int* __restrict__ bslot;  // LDS slot of each entry's x value (< 0: -(column + 1), read
+                                    // from x: entries of a row beyond the block's kMaxExt)
     const int* __restrict__ bstep;  // step of each position inside its block
     const double* __restrict__ bval;
     const double* __restrict__ b;
@@ -209,7 +238,7 @@ __global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
                     if (active) {
 #pragma unroll
                         for (int u = 0; u < UNROLL; ++u) {
-                            xv[u] = lds_get(&slot[sl[u]]);
+                            xv[u] = sl[u] >= 0 ? lds_get(&slot[sl[u]]) : poll_bits(a.x + (-sl[u] - 1));
                             ok &= u >= cnt || xv[u] != S;
                         }
                     }
@@ -255,33 +284,6 @@ int upload_b(X** dst, const std::vector<X>& src) {
     return SPTRSV_OK;
 }
 
-// Grid of a natural-order stencil factor: the distinct off-diagonal offsets are {1, nx} or
-// {1, nx, nx*ny} with n a multiple of the largest.  Returns false for anything else.
-bool detect_grid(int n, const std::vector<int>& rp, const std::vector<int>& col, int64_t* dims) {
-    std::map<int, int64_t> offs;
-    for (int r = 0; r < n; ++r)
-        for (int e = rp[r] + 1; e < rp[r + 1]; ++e) {
-            ++offs[col[e] - r];
-            if (offs.size() > 3) return false;
-        }
-    std::vector<int> o;
-    for (auto& kv : offs) o.push_back(kv.first);
-    if (o.empty() || o[0] != 1) return false;
-    if (o.size() == 2 && o[1] > 1 && n % o[1] == 0) {
-        dims[0] = o[1];
-        dims[1] = n / o[1];
-        dims[2] = 1;
-        return true;
-    }
-    if (o.size() == 3 && o[1] > 1 && o[2] > o[1] && o[2] % o[1] == 0 && n % o[2] == 0) {
-        dims[0] = o[1];
-        dims[1] = o[2] / o[1];
-        dims[2] = n / o[2];
-        return true;
-    }
-    return false;
-}
-
 // Block order (position -> row) and block boundaries.  `kind` reports which order was used:
 // 1 = grid bricks, 2 = level bands.
 void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col, const std::vector<int>& lev,
@@ -289,7 +291,7 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
     int64_t d[3];
     order.assign((size_t)n, 0);
     bptr.clear();
-    if (detect_grid(n, rp, col, d)) {
+    if (detect_grid(n, rp.data(), col.data(), d)) {
         const int64_t bx = d[2] > 1 ? 4 : 8, by = d[2] > 1 ? 4 : 8, bz = d[2] > 1 ? 4 : 1;
         const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
         // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
@@ -350,10 +352,6 @@ int build_blocks(sptrsv_plan_s* P) {
     SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
     SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
     SPT_HIP(hipMemcpy(lev.data(), P->levels, lev.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int r = 0; r < n; ++r)
-        if (rp[r + 1] - rp[r] - 1 > kMaxExt)
-            return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: row " + std::to_string(r) + " has more than " +
-                                                    std::to_string(kMaxExt) + " off-diagonal entries");
     std::vector<int> order, pre;
     int kind = 0;
     block_order(n, rp, col, lev, order, pre, &kind);
@@ -386,6 +384,8 @@ int build_blocks(sptrsv_plan_s* P) {
             cut = (int)ecol.size() - eptr.back() + fresh > kMaxExt;
         }
         if (cut) open_block(pos);
+        // a row with more external values than a block holds opens a block of its own: its
+        // entries beyond the first kMaxExt are read from x directly (slot -(column + 1))
         brp[(size_t)pos] = (int)e0;
         bslot[(size_t)e0] = 0;
         bval[(size_t)e0] = val[rp[r]];
@@ -396,13 +396,15 @@ int build_blocks(sptrsv_plan_s* P) {
             if (pc >= start) {
                 bslot[(size_t)e0] = pc - start;
                 bstep[(size_t)pos] = std::max(bstep[(size_t)pos], bstep[(size_t)pc] + 1);
-            } else {
-                if (stamp[(size_t)c] != nb) {
-                    stamp[(size_t)c] = nb;
-                    extidx[(size_t)c] = (int)ecol.size() - eptr.back();
-                    ecol.push_back(c);
-                }
+            } else if (stamp[(size_t)c] == nb) {
                 bslot[(size_t)e0] = kBlockRows + extidx[(size_t)c];
+            } else if ((int)ecol.size() - eptr.back() < kMaxExt) {
+                stamp[(size_t)c] = nb;
+                extidx[(size_t)c] = (int)ecol.size() - eptr.back();
+                ecol.push_back(c);
+                bslot[(size_t)e0] = kBlockRows + extidx[(size_t)c];
+            } else {
+                bslot[(size_t)e0] = -(c + 1);
             }
             bval[(size_t)e0] = val[e];
         }

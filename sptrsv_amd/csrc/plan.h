@@ -13,6 +13,7 @@ struct sptrsv_plan_s {
     int64_t nnz = 0;
     int n_levels = 0;
     int max_row_len = 0;
+    bool grid = false;  // off-diagonal offsets of a natural-order 2D/3D grid factor (block.hip)
 
     // CSR, natural row order, diagonal first (HBM: 4(n+1) + 12 nnz bytes)
     int* rowptr = nullptr;
@@ -154,6 +155,7 @@ void tile_free(sptrsv_plan_s* p);         // tile.hip
 int block_solve(sptrsv_plan_s* p, int precision, int occ, const double* d_b, double* d_x, int m,
                 uint64_t limit);  // block.hip
 void block_free(sptrsv_plan_s* p);        // block.hip
+bool detect_grid(int n, const int* rowptr, const int* col, int64_t* dims);  // block.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
 int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip

@@ -257,6 +257,8 @@ int sptrsv_plan_create_csr(int n, int64_t nnz, const int64_t* rowptr, const int*
     for (int i = 0; i <= n; ++i) rp32[i] = (int)rowptr[i];
     for (int i = 0; i < n; ++i) max_len = std::max(max_len, rp32[i + 1] - rp32[i]);
     p->max_row_len = max_len;
+    int64_t dims[3];
+    p->grid = detect_grid(n, rp32.data(), col, dims);
     hipError_t e = hipMemcpy(p->rowptr, rp32.data(), ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->col, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->val, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice);

@@ -679,8 +679,9 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // AUTO: one right-hand side over a factor whose levels hold fewer than four rows on average
 // (a dependency chain) goes to the single-wave chain kernel, which keeps each window of rows in
 // LDS instead of paying a device-wide round trip per level; everything else to LEVEL_PULL.
-int default_variant(const sptrsv_plan_s* P, int m) {
+int default_variant(const sptrsv_plan_s* P, int m, int precision) {
     if (m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
+    if (m == 1 && precision == SPTRSV_FP64 && P->grid) return SPTRSV_VAR_BLOCK;
     return SPTRSV_VAR_LEVEL_PULL;
 }
 
@@ -758,12 +759,19 @@ using namespace sptrsv;
 
 extern "C" {
 
+int sptrsv_plan_auto_variant(sptrsv_plan_t P, int m, int precision, int* variant) {
+    if (!P || !variant) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    *variant = default_variant(P, m, precision);
+    return SPTRSV_OK;
+}
+
 int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d_b, double* d_x, int m) {
     if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
     if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
     sptrsv_params_t prm{};
     if (params) prm = *params;
-    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
+    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m, prm.precision) : prm.variant;
     if (variant < 1 || variant >= SPTRSV_VAR_COUNT)
         return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
     const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
@@ -846,7 +854,7 @@ int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const s
     sptrsv_refine_t ro{};
     if (opts) ro = *opts;
     const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
-    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m) : prm.variant;
+    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m, low) : prm.variant;
     if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK)
         return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile and block variants have no reduced-precision form");
     if (variant == SPTRSV_VAR_CHAIN && m != 1)

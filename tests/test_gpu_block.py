@@ -58,6 +58,10 @@ CASES = {
     "chol2d_15": (gen.laplace2d_cholesky(15), 2),
     "chain_300": (offsets_factor(300, (1,), 4), 2),
     "diag_only": (gen.UpperCSR(100, np.arange(101), np.arange(100, dtype=np.int32), np.linspace(1, 2, 100)), 2),
+    # row 0 depends on all 1,999 others: more external values than a block's 512 LDS slots,
+    # the rest read from x directly
+    "long_row_2000": (offsets_factor(2000, (1,) + tuple(range(2, 2000)), 6,
+                                     keep=lambda r, o: (o == 1) | (r == 0)), 2),
 }
 
 

@@ -71,8 +71,10 @@ constexpr int kExtPerLane = 8;
 constexpr int kMaxExt = kWave * kExtPerLane;  // external values per block
 constexpr int kSlots = kBlockRows + kMaxExt;  // LDS slots: the block's rows, then its externals
 constexpr int kBand = 16;                      // levels per band of the generic block order
+constexpr int kNB = 4;                         // blocks per workgroup (one ticket each)
 
 struct BlockArgs {
+    int nblk;
     const int* __restrict__ bptr;   // nblk + 1 block boundaries (positions)
     const int* __restrict__ eptr;   // nblk + 1 offsets into ecol
     const int* __restrict__ ecol;   // external values of each block (rows of earlier blocks)
@@ -95,22 +97,28 @@ __device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int UNROLL, bool SHORT>
-__global__ __launch_bounds__(2 * kWave) void k_block(BlockArgs a) {
+// A workgroup solves NB consecutive blocks: waves 0..NB-1 are their compute waves, waves
+// NB..2NB-1 their pollers, and one ticket covers all NB (a single ticket counter serialises
+// its atomics at ~12 ns each, which bounds how many blocks per microsecond can start).
+template <int UNROLL, bool SHORT, int NB>
+__global__ __launch_bounds__(2 * NB * kWave) void k_block(BlockArgs a) {
 #pragma clang fp contract(off)
     using U = uint64_t;
     constexpr U S = Bits<double>::sentinel;
-    __shared__ U slot[kSlots];
+    __shared__ U slots[NB * kSlots];
     __shared__ int s_ticket;
     const int lane = threadIdx.x & (kWave - 1);
-    const bool poller = threadIdx.x >= kWave;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;  // wave index, uniform
+    const bool poller = w >= NB;
+    U* slot = slots + (poller ? w - NB : w) * kSlots;
     if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
     if (!poller) lds_put(&slot[lane], S);
 #pragma unroll
     for (int j = 0; j < kExtPerLane; ++j)
         if (poller) lds_put(&slot[kBlockRows + lane + kWave * j], S);
     __syncthreads();
-    const int t = s_ticket;
+    const int t = s_ticket * NB + (poller ? w - NB : w);
+    if (t >= a.nblk) return;  // a whole wave; no barrier follows
     Watchdog wd;
     wd.arm(&a.ctrl[1], a.limit);
     int idle = 0, streak = 0;
@@ -451,16 +459,17 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
     k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n);
     k_reset_ticket_block<<<1, 64, 0, P->stream>>>(P->ticket);
-    BlockArgs a{P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
+    BlockArgs a{P->n_blocks, P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
                 P->blk_val, d_b, d_x, P->ticket, limit};
-    // resident blocks per CU: dynamic LDS sized so that at most `occ` workgroups share a CU
-    // (160 KiB of LDS per CU; the kernel's own slots take kSlots * 8 bytes)
-    const int o = occ > 0 ? std::min(occ, 16) : 3;
-    const size_t lds = o >= 16 ? 0 : ((size_t)(163840 / o - kSlots * 8 - 512) & ~(size_t)255);
+    // resident blocks per CU: dynamic LDS sized so that at most occ / kNB workgroups share a
+    // CU (160 KiB of LDS per CU; the kernel's own slots take kNB * kSlots * 8 bytes)
+    const int o = std::max(1, (occ > 0 ? std::min(occ, 32) : 4) / kNB);
+    const size_t lds = o >= 8 ? 0 : ((size_t)(163840 / o - kNB * kSlots * 8 - 512) & ~(size_t)255);
+    const unsigned grid = (unsigned)((P->n_blocks + kNB - 1) / kNB);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (P->blk_maxlen <= 4) k_block<4, true><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
-    else k_block<4, false><<<P->n_blocks, 2 * kWave, lds, P->stream>>>(a);
+    if (P->blk_maxlen <= 4) k_block<4, true, kNB><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
+    else k_block<4, false, kNB><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

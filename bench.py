@@ -211,8 +211,9 @@ def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 
         d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
         T = tune.exhaustive(B, d_b, d_x, m=m, reps=2)
         feats = B.features(m)
+        mlp = sel.predict(feats, m=m)
         choices = {"fallback_ls1": np.full(B.nsys, capi.BCFG_LS1), "exhaustive": np.argmin(T, axis=1),
-                   "mlp": sel.predict(feats, m=m)}
+                   "mlp": mlp, "mlp_guarded": tune.guard_longest(U, sp, feats, mlp, m)}
         rm = {}
         walls = {name: [] for name in choices}
         for _ in range(3):  # the choices alternate, so a transient slows all of them alike
@@ -234,6 +235,7 @@ def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 
                         "sum_device_us": round(float(cost.sum()), 1), "longest_system_us": round(float(cost.max()), 1),
                         "regret": round(tune.regret(T, ch), 4)}
         rm["mlp_vs_exhaustive_throughput"] = round(rm["exhaustive"]["ms"] / rm["mlp"]["ms"], 4)
+        rm["mlp_guarded_vs_exhaustive_throughput"] = round(rm["exhaustive"]["ms"] / rm["mlp_guarded"]["ms"], 4)
         rm["mlp_accuracy"] = float((choices["mlp"] == choices["exhaustive"]).mean())
         # FP32 solves + FP64 refinement with the selector's configurations
         ch = choices["mlp"]

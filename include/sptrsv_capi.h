@@ -55,12 +55,13 @@ extern "C" {
                                      by level with a barrier between levels, dependencies inside the
                                      tile read from LDS, later tiles polled; m == 1, FP64, exact
                                      reference summation order */
-#define SPTRSV_VAR_BLOCK 8        /* one wave per block of <= 64 rows, blocks in a topological
-                                     order (4x4x4 / 8x8 bricks of a natural-order grid factor,
-                                     else bands of 16 levels), dependencies inside a block through
-                                     LDS, on other blocks polled; params.occupancy = resident
-                                     blocks per CU (0 = 3); m == 1, FP64, exact reference
-                                     summation order */
+#define SPTRSV_VAR_BLOCK 8        /* blocks of rows in a topological order, each solved in one
+                                     workgroup: 8x8x8 / 32x16 bricks of a natural-order grid factor
+                                     (8 compute waves), else 64-row chunks of bands of 16 levels;
+                                     dependencies inside a block through LDS, on other blocks
+                                     polled by a poller wave; params.occupancy = resident blocks
+                                     per CU (0 = 3 for grid bricks, 4 otherwise); m == 1, FP64,
+                                     exact reference summation order */
 #define SPTRSV_VAR_COUNT 9
 
 /* Value precision of the solve. */

@@ -1,24 +1,27 @@
-// block.hip -- block pull (m == 1, FP64): one workgroup per block of up to 64 rows, the
+// block.hip -- block pull (m == 1, FP64): blocks of rows solved inside one workgroup, the
 // block's internal dependencies handed over through LDS.
 //
 // In the row and level pull kernels every dependency is a device-scope store-then-poll round
 // trip (1-2 us), so the solve costs about one round trip per level of the dependency DAG.
-// Here the rows are cut into blocks of at most 64 rows, the blocks are taken in a topological
-// order (every dependency of a block lies in the same block or in an earlier one), and one
-// workgroup of two waves solves one block:
-//   * the compute wave holds a row per lane and reads every value it needs from LDS: the
+// Here the rows are cut into blocks, the blocks are taken in a topological order (every
+// dependency of a block lies in the same block or in an earlier one), and each block is
+// solved by W compute waves and one poller wave of one workgroup:
+//   * a compute wave holds a row per lane and reads every value it needs from LDS: the
 //     block's own rows from their slots as soon as their lanes store them, the values of
 //     earlier blocks from slots the poller fills; it never waits on global memory;
 //   * the poller wave holds the block's list of external values (at most kMaxExt) and polls
 //     the self-publishing x for them until all have arrived, copying each into its LDS slot.
+// Grid factors use 512-row blocks (8 compute waves), other factors 64-row blocks, four of them
+// per workgroup under one ticket.
 // A dependency inside the block costs an LDS round trip; only a dependency on another block
 // costs a device round trip, and the compute wave's progress never waits on the poller's.
 //
 // Blocks: for a factor whose off-diagonal offsets are those of a natural-order 2D or 3D grid
-// ({1, nx} or {1, nx, nx*ny}), 8x8 or 4x4x4 bricks ordered by brick anti-diagonal (a path
-// through a 128^3 grid crosses ~93 bricks instead of 381 levels); otherwise (or if the brick
+// ({1, nx} or {1, nx, nx*ny}), 32x16 or 8x8x8 bricks ordered by brick anti-diagonal (a path
+// through a 128^3 grid crosses ~45 bricks instead of 381 levels); otherwise (or if the brick
 // order is not topological) bands of 16 levels with rows in descending order inside a band,
-// cut into chunks of 64.  A block is cut short where its external list would exceed kMaxExt.
+// cut into chunks of the block size.  A block is cut short where its external list would
+// exceed kMaxExt.
 //
 // Each lane sums its row's products in ascending column order and divides by the diagonal,
 // the reference serial solver's arithmetic (src/core.cpp:18-32), so x is bitwise equal to it.
@@ -71,7 +74,8 @@ constexpr int kExtPerLane = 8;
 constexpr int kMaxExt = kWave * kExtPerLane;  // external values per block
 constexpr int kSlots = kBlockRows + kMaxExt;  // LDS slots: the block's rows, then its externals
 constexpr int kBand = 16;                      // levels per band of the generic block order
-constexpr int kNB = 4;                         // blocks per workgroup (one ticket each)
+constexpr int kNB = 4;                         // 64-row blocks per workgroup (one ticket each)
+constexpr int kBigRows = 8 * kWave;            // rows of a grid factor's block: 8 compute waves
 
 struct BlockArgs {
     int nblk;
@@ -100,24 +104,27 @@ __device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
 // A workgroup solves NB consecutive blocks: waves 0..NB-1 are their compute waves, waves
 // NB..2NB-1 their pollers, and one ticket covers all NB (a single ticket counter serialises
 // its atomics at ~12 ns each, which bounds how many blocks per microsecond can start).
-template <int UNROLL, bool SHORT, int NB>
-__global__ __launch_bounds__(2 * NB * kWave) void k_block(BlockArgs a) {
+template <int UNROLL, bool SHORT, int NB, int W>
+__global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
 #pragma clang fp contract(off)
     using U = uint64_t;
     constexpr U S = Bits<double>::sentinel;
-    __shared__ U slots[NB * kSlots];
+    constexpr int R = W * kWave;  // rows per block
+    __shared__ U slots[NB * (R + kMaxExt)];
     __shared__ int s_ticket;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;  // wave index, uniform
-    const bool poller = w >= NB;
-    U* slot = slots + (poller ? w - NB : w) * kSlots;
+    const bool poller = w >= NB * W;
+    const int blk = poller ? w - NB * W : w / W;
+    const int sw = poller ? 0 : w % W;  // compute wave of its block
+    U* slot = slots + blk * (R + kMaxExt);
     if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
-    if (!poller) lds_put(&slot[lane], S);
+    if (!poller) lds_put(&slot[sw * kWave + lane], S);
 #pragma unroll
     for (int j = 0; j < kExtPerLane; ++j)
-        if (poller) lds_put(&slot[kBlockRows + lane + kWave * j], S);
+        if (poller) lds_put(&slot[R + lane + kWave * j], S);
     __syncthreads();
-    const int t = s_ticket * NB + (poller ? w - NB : w);
+    const int t = s_ticket * NB + blk;
     if (t >= a.nblk) return;  // a whole wave; no barrier follows
     Watchdog wd;
     wd.arm(&a.ctrl[1], a.limit);
@@ -143,7 +150,7 @@ __global__ __launch_bounds__(2 * NB * kWave) void k_block(BlockArgs a) {
 #pragma unroll
             for (int j = 0; j < kExtPerLane; ++j) {
                 if (need[j] && v[j] != S) {
-                    lds_put(&slot[kBlockRows + lane + kWave * j], v[j]);
+                    lds_put(&slot[R + lane + kWave * j], v[j]);
                     need[j] = false;
                     got = true;
                 }
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(2 * NB * kWave) void k_block(BlockArgs a) {
     // reads was stored at an earlier step of this same wave; only its external values may
     // still be missing, and the lanes of step s wait for those in LDS.
     const int p0 = a.bptr[t], p1 = a.bptr[t + 1];
-    const int pos = p0 + lane;
+    const int pos = p0 + sw * kWave + lane;
     const bool valid = pos < p1;
     int row = 0, step = -1, p = 0, end = 0;
     double diag = 1.0, bv = 0.0;
@@ -269,9 +276,11 @@ __global__ __launch_bounds__(2 * NB * kWave) void k_block(BlockArgs a) {
         }
         if (mine && !dead) {
             const double xi = (bv - acc) / diag;
-            lds_put(&slot[lane], __builtin_bit_cast(U, xi));
+            lds_put(&slot[sw * kWave + lane], __builtin_bit_cast(U, xi));
             publish(a.x + row, xi);
         }
+        // the block's other compute waves may sleep on this step's values
+        if (W > 1 && __any(mine)) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_wakeup" ::: "memory");
     }
 }
 
@@ -295,12 +304,16 @@ int upload_b(X** dst, const std::vector<X>& src) {
 // Block order (position -> row) and block boundaries.  `kind` reports which order was used:
 // 1 = grid bricks, 2 = level bands.
 void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col, const std::vector<int>& lev,
-                 std::vector<int>& order, std::vector<int>& bptr, int* kind) {
+                 int R, std::vector<int>& order, std::vector<int>& bptr, int* kind) {
     int64_t d[3];
     order.assign((size_t)n, 0);
     bptr.clear();
     if (detect_grid(n, rp.data(), col.data(), d)) {
-        const int64_t bx = d[2] > 1 ? 4 : 8, by = d[2] > 1 ? 4 : 8, bz = d[2] > 1 ? 4 : 1;
+        // 3D: 4x4x4 (64 rows), 8x8x4 (256) or 8x8x8 (512); 2D: 8x8, 16x16 or 32x16
+        const bool big = R > kBlockRows;
+        const int64_t bx = d[2] > 1 ? (big ? 8 : 4) : (R >= 512 ? 32 : big ? 16 : 8);
+        const int64_t by = d[2] > 1 ? bx : (big ? 16 : 8);
+        const int64_t bz = d[2] > 1 ? (R >= 512 ? 8 : 4) : 1;
         const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
         // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
         // coordinates), then descending brick index; rows inside a brick descending
@@ -318,7 +331,7 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
         });
         for (int pos = 0; pos < n; ++pos)
             if (pos == 0 || key[(size_t)order[(size_t)pos]] != key[(size_t)order[(size_t)pos - 1]] ||
-                pos - bptr.back() == kBlockRows)
+                pos - bptr.back() == R)
                 bptr.push_back(pos);
         bptr.push_back(n);
         // topological check: every dependency in the same or an earlier block
@@ -345,7 +358,7 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
         const int bx = (lev[(size_t)x] - 1) / kBand, by = (lev[(size_t)y] - 1) / kBand;
         return bx != by ? bx < by : x > y;
     });
-    for (int pos = 0; pos < n; pos += kBlockRows) bptr.push_back(pos);
+    for (int pos = 0; pos < n; pos += R) bptr.push_back(pos);
     bptr.push_back(n);
     *kind = 2;
 }
@@ -362,7 +375,10 @@ int build_blocks(sptrsv_plan_s* P) {
     SPT_HIP(hipMemcpy(lev.data(), P->levels, lev.size() * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<int> order, pre;
     int kind = 0;
-    block_order(n, rp, col, lev, order, pre, &kind);
+    // grid factors: 8x8x8 bricks (32x16 in 2D) on 8 compute waves each; otherwise 64-row blocks
+    const int R = P->grid ? kBigRows : kBlockRows;
+    P->blk_rows = R;
+    block_order(n, rp, col, lev, R, order, pre, &kind);
     std::vector<int> posof((size_t)n);
     for (int pos = 0; pos < n; ++pos) posof[(size_t)order[(size_t)pos]] = pos;
     // cut blocks where the external list would exceed kMaxExt; external lists in order of
@@ -405,12 +421,12 @@ int build_blocks(sptrsv_plan_s* P) {
                 bslot[(size_t)e0] = pc - start;
                 bstep[(size_t)pos] = std::max(bstep[(size_t)pos], bstep[(size_t)pc] + 1);
             } else if (stamp[(size_t)c] == nb) {
-                bslot[(size_t)e0] = kBlockRows + extidx[(size_t)c];
+                bslot[(size_t)e0] = R + extidx[(size_t)c];
             } else if ((int)ecol.size() - eptr.back() < kMaxExt) {
                 stamp[(size_t)c] = nb;
                 extidx[(size_t)c] = (int)ecol.size() - eptr.back();
                 ecol.push_back(c);
-                bslot[(size_t)e0] = kBlockRows + extidx[(size_t)c];
+                bslot[(size_t)e0] = R + extidx[(size_t)c];
             } else {
                 bslot[(size_t)e0] = -(c + 1);
             }
@@ -461,15 +477,24 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     k_reset_ticket_block<<<1, 64, 0, P->stream>>>(P->ticket);
     BlockArgs a{P->n_blocks, P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
                 P->blk_val, d_b, d_x, P->ticket, limit};
-    // resident blocks per CU: dynamic LDS sized so that at most occ / kNB workgroups share a
-    // CU (160 KiB of LDS per CU; the kernel's own slots take kNB * kSlots * 8 bytes)
-    const int o = std::max(1, (occ > 0 ? std::min(occ, 32) : 4) / kNB);
-    const size_t lds = o >= 8 ? 0 : ((size_t)(163840 / o - kNB * kSlots * 8 - 512) & ~(size_t)255);
-    const unsigned grid = (unsigned)((P->n_blocks + kNB - 1) / kNB);
+    // resident workgroups per CU: dynamic LDS sized so that at most that many share a CU (160 KiB
+    // of LDS per CU).  64-row blocks: kNB blocks per workgroup, `occ` blocks per CU (default
+    // 4); 512-row blocks: one per workgroup, `occ` per CU (default 3)
+    const bool big = P->blk_rows == kBigRows;
+    const int nb = big ? 1 : kNB;
+    const int per_cu = std::max(1, (occ > 0 ? std::min(occ, 32) : big ? 3 : 4) / nb);
+    const size_t own = (size_t)nb * (P->blk_rows + kMaxExt) * 8 + 512;
+    const size_t lds = per_cu >= 8 ? 0 : ((163840 / per_cu - own) & ~(size_t)255);
+    const unsigned grid = (unsigned)((P->n_blocks + nb - 1) / nb);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (P->blk_maxlen <= 4) k_block<4, true, kNB><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
-    else k_block<4, false, kNB><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
+    if (big) {
+        if (P->blk_maxlen <= 4) k_block<4, true, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
+        else k_block<4, false, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    } else {
+        if (P->blk_maxlen <= 4) k_block<4, true, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
+        else k_block<4, false, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
+    }
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

@@ -109,6 +109,7 @@ struct sptrsv_plan_s {
     int n_blocks = 0;
     int blk_kind = 0;
     int blk_maxlen = 0;          // most off-diagonal entries in a row
+    int blk_rows = 0;            // rows per block the layout was built for (64 or 256)
     int* blk_ptr = nullptr;
     int* blk_eptr = nullptr;     // per block: its external values blk_ecol[blk_eptr[k] ..)
     int* blk_ecol = nullptr;

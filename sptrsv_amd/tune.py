@@ -5,6 +5,9 @@
 * Selector: the 24 features of a system -> the configuration, by a 24-64-64-K ReLU network
   trained on the GPU (sptrsv_mlp_*) on the autotuner's own labels (argmin time).
 * lpt_order(): longest-predicted-time-first queue order for the persistent launch.
+* guard_longest(): re-decides the few systems that bound the launch's makespan (most levels,
+  most nonzeros) by timing them alone under every configuration: a persistent launch cannot end
+  before its longest system, so one wrong pick there costs the whole batch.
 """
 from __future__ import annotations
 
@@ -78,6 +81,46 @@ class Selector:
 
     def close(self):
         self.mlp.close()
+
+
+def sub_batch(U, sys_ptr, idx):
+    """systems `idx` of a block-diagonal batch as a batch of their own: (CSR, sys_ptr)"""
+    from .gen import UpperCSR
+    rp, col, val = np.asarray(U.rowptr, dtype=np.int64), np.asarray(U.col), np.asarray(U.val)
+    sp = np.asarray(sys_ptr, dtype=np.int64)
+    rps, cols, vals, ptr = [np.zeros(1, dtype=np.int64)], [], [], [0]
+    for s in idx:
+        r0, r1 = int(sp[s]), int(sp[s + 1])
+        e0, e1 = int(rp[r0]), int(rp[r1])
+        rps.append(rp[r0 + 1:r1 + 1] - e0 + rps[-1][-1])
+        cols.append(col[e0:e1].astype(np.int64) - r0 + ptr[-1])
+        vals.append(val[e0:e1])
+        ptr.append(ptr[-1] + r1 - r0)
+    n = ptr[-1]
+    sub = UpperCSR(n, np.concatenate(rps), np.concatenate(cols).astype(np.int32), np.concatenate(vals), None, "sub")
+    return sub, np.asarray(ptr, dtype=np.int32)
+
+
+def guard_longest(U, sys_ptr, feats: np.ndarray, choice: np.ndarray, m: int, k: int = 8) -> np.ndarray:
+    """The selector's choices with the k systems of most levels and the k of most nonzeros
+    (features 6 and 1: log2 levels, log2 nnz) re-decided by timing them alone, in a batch of
+    their own, under every configuration valid at m."""
+    feats = np.asarray(feats)
+    idx = np.unique(np.concatenate([np.argsort(-feats[:, 6], kind="stable")[:k],
+                                    np.argsort(-feats[:, 1], kind="stable")[:k]]))
+    S, sp = sub_batch(U, sys_ptr, idx)
+    B = capi.Batch(S, sp)
+    b = np.ones((S.n, m)) if m > 1 else np.ones(S.n)
+    d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+    try:
+        T = exhaustive(B, d_b, d_x, m=m, reps=2)
+    finally:
+        d_b.free()
+        d_x.free()
+        B.close()
+    out = np.array(choice, dtype=np.int32, copy=True)
+    out[idx] = np.argmin(T, axis=1)
+    return out
 
 
 def regret(times: np.ndarray, choice: np.ndarray) -> float:

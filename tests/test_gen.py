@@ -129,3 +129,20 @@ def test_to_csc_matches_oracle_transpose():
     cp, ri, cv = gen.to_csc(U)
     ocp, ori, ocv = O.transpose(U.rowptr, U.col, U.val)
     assert np.array_equal(cp, ocp) and np.array_equal(ri, ori) and np.array_equal(cv, ocv)
+
+
+def test_sub_batch_extracts_systems():
+    """tune.sub_batch: the chosen systems of a block-diagonal batch, renumbered, same entries"""
+    from sptrsv_amd import tune
+    U, sp, kinds = gen.batch_systems(40, seed=3, nmin=64, nmax=800)
+    idx = [0, 7, 39]
+    S, p = tune.sub_batch(U, sp, idx)
+    assert p[0] == 0 and p[-1] == S.n and S.rowptr[-1] == S.nnz
+    rp = np.asarray(U.rowptr)
+    for j, s in enumerate(idx):
+        r0, r1, q0, q1 = sp[s], sp[s + 1], p[j], p[j + 1]
+        assert r1 - r0 == q1 - q0
+        e0, e1, f0, f1 = rp[r0], rp[r1], S.rowptr[q0], S.rowptr[q1]
+        assert np.array_equal(np.asarray(U.col)[e0:e1] - r0, np.asarray(S.col)[f0:f1] - q0)
+        assert np.array_equal(np.asarray(U.val)[e0:e1], np.asarray(S.val)[f0:f1])
+        assert np.array_equal(np.diff(rp[r0:r1 + 1]), np.diff(np.asarray(S.rowptr)[q0:q1 + 1]))

@@ -133,6 +133,22 @@ def test_time_config_and_exhaustive(small_batch):
     assert np.all(np.isfinite(T[:, :capi.BCFG_CHAIN])) and np.all(T[:, :capi.BCFG_CHAIN] > 0)
 
 
+def test_guard_longest_redecides_only_the_longest(small_batch):
+    """the makespan guard changes at most the systems of most levels / most nonzeros, and its
+    picks are configurations valid at m"""
+    U, sp, kinds, B = small_batch
+    feats = B.features(4)
+    base = np.zeros(B.nsys, dtype=np.int32)
+    g = tune.guard_longest(U, sp, feats, base, 4, k=3)
+    guarded = set(np.argsort(-feats[:, 6], kind="stable")[:3]) | set(np.argsort(-feats[:, 1], kind="stable")[:3])
+    changed = set(np.nonzero(g != base)[0])
+    assert changed <= guarded
+    assert np.all((g >= 0) & (g < capi.BCFG_COUNT))
+    xt, b = gen.rhs_for(U, 4, seed=8, layout="row")
+    B.set_configs(g, tune.lpt_order(np.ones(B.nsys)))
+    assert O.rel_error(oracle_rowmajor(U, b), solve_batch(B, b, 4)) <= FP64_TOL
+
+
 # ---- the selector network ---------------------------------------------------------------------
 
 def test_mlp_gradient_matches_fp64_reference():

@@ -1,7 +1,8 @@
-"""BLOCK variant: one wave per block of <= 64 rows with the block's internal dependencies passed
-through LDS.  Checks the block order the library builds (a partition into blocks of 1..64 rows
-whose every dependency lies in the same or an earlier block; 4x4x4 / 8x8 bricks for
-natural-order grid factors, level bands otherwise) on the CPU from the arrays it returns, and
+"""BLOCK variant: blocks of rows solved inside one workgroup with the block's internal
+dependencies passed through LDS.  Checks the block order the library builds (a partition into
+blocks whose every dependency lies in the same or an earlier block; 8x8x8 / 32x16 bricks for
+natural-order grid factors, 64-row chunks of level bands otherwise) on the CPU from the arrays
+it returns, and
 FP64 x bitwise equal to the serial oracle (the kernel keeps the reference's summation order)."""
 import numpy as np
 import pytest
@@ -51,9 +52,9 @@ CASES = {
     "grid3d_9x6x5": (grid3d(9, 6, 5), 1),
     "grid2d_20x13": (grid2d(20, 13), 1),
     "grid2d_8x8": (grid2d(8, 8), 1),             # one brick
-    # offsets {1, 16} with the +1 entries wrapping from line to line: the brick order is not
-    # topological, so the level-band order is used
-    "wrap_16x20": (offsets_factor(16 * 20, (1, 16), 3), 2),
+    # offsets {1, 64} with the +1 entries wrapping from line to line across two 32-wide bricks:
+    # the brick order is not topological, so the level-band order is used
+    "wrap_64x20": (offsets_factor(64 * 20, (1, 64), 3), 2),
     "zipf_3000": (gen.random_zipf(3000, seed=11), 2),
     "chol2d_15": (gen.laplace2d_cholesky(15), 2),
     "chain_300": (offsets_factor(300, (1,), 4), 2),
@@ -69,7 +70,7 @@ def check_order(U, kind, ptr, perm):
     n = U.n
     assert ptr[0] == 0 and ptr[-1] == n
     sizes = np.diff(ptr)
-    assert sizes.min() >= 1 and sizes.max() <= 64
+    assert sizes.min() >= 1 and sizes.max() <= 512  # 512 for grid factors, 64 otherwise
     assert np.array_equal(np.sort(perm), np.arange(n))
     blk = np.empty(n, dtype=np.int64)
     blk[perm] = np.repeat(np.arange(len(sizes)), sizes)
@@ -96,19 +97,31 @@ def test_block_order_and_parity(name):
 
 
 def test_bricks_of_ic0_12():
-    """12^3 grid: 27 bricks of 4x4x4 rows, each block one brick"""
+    """12^3 grid: 8 bricks of up to 8x8x8 rows (8 or 4 along each axis), each block one brick"""
     U = CASES["ic0_12"][0]
     with capi.Plan.from_upper(U) as P:
         kind, ptr, perm = P.blocks()
-    assert kind == 1 and len(ptr) == 28 and np.all(np.diff(ptr) == 64)
+    assert kind == 1 and len(ptr) == 9
     N = 12
-    for b in range(27):
+    sizes = []
+    for b in range(8):
         r = perm[ptr[b]:ptr[b + 1]]
-        cells = {(int(i) // 4, int(j) // 4, int(k) // 4) for i, j, k in zip(r % N, (r // N) % N, r // (N * N))}
+        cells = {(int(i) // 8, int(j) // 8, int(k) // 8) for i, j, k in zip(r % N, (r // N) % N, r // (N * N))}
         assert len(cells) == 1
-    # bricks by descending anti-diagonal: the first block is the far corner brick
+        sizes.append(len(r))
+    assert sorted(sizes) == [64, 128, 128, 128, 256, 256, 256, 512]
+    # bricks by descending anti-diagonal: the first block is the far corner brick (4x4x4)
     r = perm[:64]
-    assert np.all(r % N >= 8) and np.all((r // N) % N >= 8) and np.all(r // (N * N) >= 8)
+    assert ptr[1] == 64 and np.all(r % N >= 8) and np.all((r // N) % N >= 8) and np.all(r // (N * N) >= 8)
+
+
+def test_chunks_of_a_general_factor():
+    """a factor without grid structure: bands of levels cut into 64-row chunks"""
+    U = CASES["zipf_3000"][0]
+    with capi.Plan.from_upper(U) as P:
+        kind, ptr, perm = P.blocks()
+    sizes = np.diff(ptr)
+    assert kind == 2 and sizes.max() <= 64 and np.median(sizes) == 64
 
 
 def test_block_rejects():

@@ -72,7 +72,6 @@ namespace {
 constexpr int kBlockRows = 64;
 constexpr int kExtPerLane = 8;
 constexpr int kMaxExt = kWave * kExtPerLane;  // external values per block
-constexpr int kSlots = kBlockRows + kMaxExt;  // LDS slots: the block's rows, then its externals
 constexpr int kBand = 16;                      // levels per band of the generic block order
 constexpr int kNB = 4;                         // 64-row blocks per workgroup (one ticket each)
 constexpr int kBigRows = 8 * kWave;            // rows of a grid factor's block: 8 compute waves
@@ -101,9 +100,10 @@ __device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// A workgroup solves NB consecutive blocks: waves 0..NB-1 are their compute waves, waves
-// NB..2NB-1 their pollers, and one ticket covers all NB (a single ticket counter serialises
-// its atomics at ~12 ns each, which bounds how many blocks per microsecond can start).
+// A workgroup solves NB consecutive blocks of W * 64 rows: waves 0..NB*W-1 are the compute
+// waves (W per block, 64 rows each), waves NB*W..NB*W+NB-1 the blocks' pollers, and one
+// ticket covers all NB blocks (a single ticket counter serialises its atomics at ~12 ns each,
+// which bounds how many blocks per microsecond can start).
 template <int UNROLL, bool SHORT, int NB, int W>
 __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
 #pragma clang fp contract(off)
@@ -172,8 +172,9 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     }
     // compute wave: rows in steps; a row's step is one more than the largest step among its
     // dependencies inside the block (0 without any), so at step s every in-block value it
-    // reads was stored at an earlier step of this same wave; only its external values may
-    // still be missing, and the lanes of step s wait for those in LDS.
+    // reads belongs to an earlier step: stored already by this wave, or by another compute wave
+    // of the block that may still be behind.  The lanes of step s wait in LDS for whatever is
+    // still missing (another wave's rows, external values).
     const int p0 = a.bptr[t], p1 = a.bptr[t + 1];
     const int pos = p0 + sw * kWave + lane;
     const bool valid = pos < p1;
@@ -309,11 +310,11 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
     order.assign((size_t)n, 0);
     bptr.clear();
     if (detect_grid(n, rp.data(), col.data(), d)) {
-        // 3D: 4x4x4 (64 rows), 8x8x4 (256) or 8x8x8 (512); 2D: 8x8, 16x16 or 32x16
-        const bool big = R > kBlockRows;
-        const int64_t bx = d[2] > 1 ? (big ? 8 : 4) : (R >= 512 ? 32 : big ? 16 : 8);
+        // 3D: 8x8x8 (512 rows) or 4x4x4 (64); 2D: 32x16 or 8x8
+        const bool big = R >= kBigRows;
+        const int64_t bx = d[2] > 1 ? (big ? 8 : 4) : (big ? 32 : 8);
         const int64_t by = d[2] > 1 ? bx : (big ? 16 : 8);
-        const int64_t bz = d[2] > 1 ? (R >= 512 ? 8 : 4) : 1;
+        const int64_t bz = d[2] > 1 ? bx : 1;
         const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
         // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
         // coordinates), then descending brick index; rows inside a brick descending

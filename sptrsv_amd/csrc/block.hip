@@ -285,14 +285,12 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     }
 }
 
-__global__ void k_fill_sentinel_b(double* x, int64_t n) {
+// x := sentinel, and the ticket counter reset (one launch before the solve instead of two)
+__global__ void k_fill_sentinel_b(double* x, int64_t n, int* ctrl) {
     const uint64_t s = Bits<double>::sentinel;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[0] = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         reinterpret_cast<uint64_t*>(x)[i] = s;
-}
-
-__global__ void k_reset_ticket_block(int* ctrl) {
-    if (threadIdx.x == 0) ctrl[0] = 0;
 }
 
 template <typename X>
@@ -474,8 +472,7 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     if (P->n_blocks == 0) SPT_CHECK(build_blocks(P));
     const int64_t n = P->n;
     const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-    k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n);
-    k_reset_ticket_block<<<1, 64, 0, P->stream>>>(P->ticket);
+    k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n, P->ticket);
     BlockArgs a{P->n_blocks, P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
                 P->blk_val, d_b, d_x, P->ticket, limit};
     // resident workgroups per CU: dynamic LDS sized so that at most that many share a CU (160 KiB

@@ -149,3 +149,20 @@ def test_block_transposed():
     A = sp.csr_matrix((U.val, U.col, U.rowptr), shape=(U.n, U.n))
     ref = spsolve_triangular(A.T.tocsr(), b, lower=True)
     assert O.rel_error(ref, got) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["chain_4000", "grid3d_8x8x1600"])
+def test_block_watchdog_reports_and_recovers(name):
+    """A 1 us per-wait limit forces the block kernel's own bounded waits (poller and compute
+    waves) to give up, every wave exiting normally; the host sees ERR_DEADLOCK, and the next
+    solve is bitwise right.  A 4000-row chain (64-row blocks, one after the other) and an
+    8x8x1600 grid (200 bricks of 8x8x8 in one chain along z: the later ones wait ~1 ms)."""
+    U = offsets_factor(4000, (1,), 12) if name == "chain_4000" else grid3d(8, 8, 1600, seed=13)
+    banner(f"block watchdog {name}")
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 1, seed=9)
+        with pytest.raises(capi.SptrsvError) as e:
+            P.solve_host(b, capi.params(variant=capi.VAR_BLOCK, watchdog_us=1))
+        assert e.value.code == capi.ERR_DEADLOCK
+        x = P.solve_host(b, capi.params(variant=capi.VAR_BLOCK))
+        assert np.array_equal(x, O.base_gauss_upper(U.rowptr, U.col, U.val, b)[0])

@@ -30,7 +30,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -43,14 +42,16 @@ namespace sptrsv {
 // Grid of a natural-order stencil factor: the distinct off-diagonal offsets are {1, nx} or
 // {1, nx, nx*ny} with n a multiple of the largest.  Returns false for anything else.
 bool detect_grid(int n, const int* rp, const int* col, int64_t* dims) {
-    std::map<int, int64_t> offs;
+    int seen[3], k = 0;  // distinct offsets so far (one scan, at most three kept)
     for (int r = 0; r < n; ++r)
         for (int e = rp[r] + 1; e < rp[r + 1]; ++e) {
-            ++offs[col[e] - r];
-            if (offs.size() > 3) return false;
+            const int d = col[e] - r;
+            if ((k > 0 && d == seen[0]) || (k > 1 && d == seen[1]) || (k > 2 && d == seen[2])) continue;
+            if (k == 3) return false;
+            seen[k++] = d;
         }
-    std::vector<int> o;
-    for (auto& kv : offs) o.push_back(kv.first);
+    std::vector<int> o(seen, seen + k);
+    std::sort(o.begin(), o.end());
     if (o.empty() || o[0] != 1) return false;
     if (o.size() == 2 && o[1] > 1 && n % o[1] == 0) {
         dims[0] = o[1];

@@ -11,9 +11,9 @@ drop-in entry point of that name runs:
     custom       supernodal_upper          SUPERNODAL (supernodes from the .snodes file)
     blas         supernodal_blas_upper     SUPERNODAL (the padded layout solves on the same kernel)
     barrier      barrier_upper             LEVELSET (one launch per level)
-    syncfree     sptrsv_syncfree_opencl    AUTO
+    syncfree     sptrsv_syncfree_opencl    AUTO (BLOCK on natural-order grid factors)
     write_first  sptrsv_syncfree3_opencl   ROW_PULL
-    level_pull / tile / chain / csc_push    this build's other kernels
+    block / level_pull / tile / chain / csc_push    this build's other kernels
 
 A process prints "Time: <seconds>" last on stderr, like the reference CLI (src/utils.cpp:353),
 where the time is the device solve time of one solve with data resident on the GPU (the
@@ -38,7 +38,7 @@ ALGOS = ["base", "custom", "blas", "barrier", "syncfree", "write_first"]
 def _variant(capi, algo):
     table = {"base": capi.VAR_ROW_PULL, "custom": capi.VAR_SUPERNODAL, "blas": capi.VAR_SUPERNODAL,
              "barrier": capi.VAR_LEVELSET, "syncfree": capi.VAR_AUTO, "write_first": capi.VAR_ROW_PULL,
-             "level_pull": capi.VAR_LEVEL_PULL, "tile": capi.VAR_TILE, "chain": capi.VAR_CHAIN,
+             "level_pull": capi.VAR_LEVEL_PULL, "tile": capi.VAR_TILE, "chain": capi.VAR_CHAIN, "block": capi.VAR_BLOCK,
              "csc_push": capi.VAR_CSC_PUSH}
     if algo not in table:
         raise SystemExit(f"unknown algorithm {algo}; one of {', '.join(table)}")

@@ -316,7 +316,10 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
         const int64_t bz = d[2] > 1 ? bx : 1;
         const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
         // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
-        // coordinates), then descending brick index; rows inside a brick descending
+        // coordinates), then descending brick index; rows inside a brick by level, then
+        // descending: each compute wave then holds a run of consecutive levels, so most of its
+        // rows' in-brick dependencies are its own rows (by plane, every step would wait on the
+        // wave of the neighbouring plane)
         std::vector<int64_t> key((size_t)n);
         for (int r = 0; r < n; ++r) {
             const int64_t i = r % d[0], j = (r / d[0]) % d[1], k = r / (d[0] * d[1]);
@@ -327,7 +330,9 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
         }
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-            return key[(size_t)x] != key[(size_t)y] ? key[(size_t)x] < key[(size_t)y] : x > y;
+            if (key[(size_t)x] != key[(size_t)y]) return key[(size_t)x] < key[(size_t)y];
+            if (lev[(size_t)x] != lev[(size_t)y]) return lev[(size_t)x] < lev[(size_t)y];
+            return x > y;
         });
         for (int pos = 0; pos < n; ++pos)
             if (pos == 0 || key[(size_t)order[(size_t)pos]] != key[(size_t)order[(size_t)pos - 1]] ||

@@ -14,14 +14,15 @@
 // Grid factors use 512-row blocks (8 compute waves), other factors 64-row blocks, four of them
 // per workgroup under one ticket.
 // A dependency inside the block costs an LDS round trip; only a dependency on another block
-// costs a device round trip, and the compute wave's progress never waits on the poller's.
+// costs a device round trip (the poller's), and a compute wave never waits on a memory round
+// trip of its own.
 //
 // Blocks: for a factor whose off-diagonal offsets are those of a natural-order 2D or 3D grid
 // ({1, nx} or {1, nx, nx*ny}), 32x16 or 8x8x8 bricks ordered by brick anti-diagonal (a path
-// through a 128^3 grid crosses ~45 bricks instead of 381 levels); otherwise (or if the brick
-// order is not topological) bands of 16 levels with rows in descending order inside a band,
-// cut into chunks of the block size.  A block is cut short where its external list would
-// exceed kMaxExt.
+// through a 128^3 grid crosses ~45 bricks instead of 381 levels), rows inside a brick by level;
+// otherwise (or if the brick order is not topological) bands of 16 levels with rows in
+// descending order inside a band, cut into chunks of the block size.  A block is cut short
+// where its external list would exceed kMaxExt.
 //
 // Each lane sums its row's products in ascending column order and divides by the diagonal,
 // the reference serial solver's arithmetic (src/core.cpp:18-32), so x is bitwise equal to it.

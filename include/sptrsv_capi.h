@@ -60,8 +60,9 @@ extern "C" {
                                      (8 compute waves), else 64-row chunks of bands of 16 levels;
                                      dependencies inside a block through LDS, on other blocks
                                      polled by a poller wave; params.occupancy = resident blocks
-                                     per CU (0 = 3 for grid bricks, 4 otherwise); m == 1, FP64,
-                                     exact reference summation order */
+                                     per CU (0 = 3 for grid bricks, 4 otherwise); FP64, m <= 4 on
+                                     grid factors, m == 1 otherwise; exact reference summation
+                                     order */
 #define SPTRSV_VAR_COUNT 9
 
 /* Value precision of the solve. */
@@ -157,8 +158,8 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
                         double* val /* nnz */);
 
 /* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
- * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average, BLOCK for FP64 m == 1
- * on a natural-order grid factor, LEVEL_PULL otherwise. */
+ * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average, BLOCK for FP64
+ * m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise. */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
 
 /* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).

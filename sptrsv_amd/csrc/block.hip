@@ -1,5 +1,6 @@
-// block.hip -- block pull (m == 1, FP64): blocks of rows solved inside one workgroup, the
-// block's internal dependencies handed over through LDS.
+// block.hip -- block pull (FP64; m == 1, or up to 4 right-hand sides on grid factors): blocks
+// of rows solved inside one workgroup, the block's internal dependencies handed over through
+// LDS.
 //
 // In the row and level pull kernels every dependency is a device-scope store-then-poll round
 // trip (1-2 us), so the solve costs about one round trip per level of the dependency DAG.
@@ -77,9 +78,11 @@ constexpr int kMaxExt = kWave * kExtPerLane;  // external values per block
 constexpr int kBand = 16;                      // levels per band of the generic block order
 constexpr int kNB = 4;                         // 64-row blocks per workgroup (one ticket each)
 constexpr int kBigRows = 8 * kWave;            // rows of a grid factor's block: 8 compute waves
+constexpr int kBlockMaxM = 4;                  // right-hand sides per lane (grid factors)
 
 struct BlockArgs {
     int nblk;
+    int m;                          // right-hand sides (row stride of b and x)
     const int* __restrict__ bptr;   // nblk + 1 block boundaries (positions)
     const int* __restrict__ eptr;   // nblk + 1 offsets into ecol
     const int* __restrict__ ecol;   // external values of each block (rows of earlier blocks)
@@ -106,25 +109,35 @@ __device__ __forceinline__ void lds_put(uint64_t* p, uint64_t v) {
 // waves (W per block, 64 rows each), waves NB*W..NB*W+NB-1 the blocks' pollers, and one
 // ticket covers all NB blocks (a single ticket counter serialises its atomics at ~12 ns each,
 // which bounds how many blocks per microsecond can start).
-template <int UNROLL, bool SHORT, int NB, int W>
+// M > 1: M right-hand sides per lane, the block's LDS slots held once per column (column c's
+// slots at c * (R + XM), so consecutive lanes read consecutive banks whatever M); the poller
+// then keeps at most XJ = 4 external rows per lane (256 per block).
+template <int UNROLL, bool SHORT, int NB, int W, int M = 1>
 __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
 #pragma clang fp contract(off)
     using U = uint64_t;
     constexpr U S = Bits<double>::sentinel;
-    constexpr int R = W * kWave;  // rows per block
-    __shared__ U slots[NB * (R + kMaxExt)];
+    constexpr int R = W * kWave;                     // rows per block
+    constexpr int XJ = M == 1 ? kExtPerLane : 4;     // external rows per poller lane
+    constexpr int XM = XJ * kWave;                   // external rows per block
+    __shared__ U slots[NB * (R + XM) * M];
     __shared__ int s_ticket;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x) / kWave;  // wave index, uniform
     const bool poller = w >= NB * W;
     const int blk = poller ? w - NB * W : w / W;
     const int sw = poller ? 0 : w % W;  // compute wave of its block
-    U* slot = slots + blk * (R + kMaxExt);
+    U* slot = slots + blk * (R + XM) * M;
+    const size_t mm = M == 1 ? 1 : (size_t)a.m;
     if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
-    if (!poller) lds_put(&slot[sw * kWave + lane], S);
 #pragma unroll
-    for (int j = 0; j < kExtPerLane; ++j)
-        if (poller) lds_put(&slot[R + lane + kWave * j], S);
+    for (int k = 0; k < M; ++k)
+        if (!poller) lds_put(&slot[k * (R + XM) + sw * kWave + lane], S);
+#pragma unroll
+    for (int j = 0; j < XJ; ++j)
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (poller) lds_put(&slot[k * (R + XM) + R + lane + kWave * j], S);
     __syncthreads();
     const int t = s_ticket * NB + blk;
     if (t >= a.nblk) return;  // a whole wave; no barrier follows
@@ -133,30 +146,38 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     int idle = 0, streak = 0;
     if (poller) {
         const int e0 = a.eptr[t], ne = a.eptr[t + 1] - e0;
-        int ec[kExtPerLane];
-        bool need[kExtPerLane];
-        U v[kExtPerLane];
+        int ec[XJ];
+        bool need[XJ][M];
+        U v[XJ][M];
 #pragma unroll
-        for (int j = 0; j < kExtPerLane; ++j) {
+        for (int j = 0; j < XJ; ++j) {
             const int k = lane + kWave * j;
-            need[j] = k < ne;
             ec[j] = a.ecol[e0 + k];  // ecol is padded by kMaxExt entries: always in bounds
-            v[j] = S;
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                need[j][c] = k < ne && (M == 1 || c < a.m);
+                v[j][c] = S;
+            }
         }
         wait_vmem();
         while (true) {
 #pragma unroll
-            for (int j = 0; j < kExtPerLane; ++j)
-                if (need[j]) v[j] = poll_bits(a.x + ec[j]);
+            for (int j = 0; j < XJ; ++j)
+#pragma unroll
+                for (int c = 0; c < M; ++c)
+                    if (need[j][c]) v[j][c] = poll_bits(a.x + (size_t)ec[j] * mm + c);
             bool got = false, left = false;
 #pragma unroll
-            for (int j = 0; j < kExtPerLane; ++j) {
-                if (need[j] && v[j] != S) {
-                    lds_put(&slot[R + lane + kWave * j], v[j]);
-                    need[j] = false;
-                    got = true;
+            for (int j = 0; j < XJ; ++j) {
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    if (need[j][c] && v[j][c] != S) {
+                        lds_put(&slot[c * (R + XM) + R + lane + kWave * j], v[j][c]);
+                        need[j][c] = false;
+                        got = true;
+                    }
+                    left |= need[j][c];
                 }
-                left |= need[j];
             }
             // wake the compute wave, which sleeps while the values of its step are missing
             if (__any(got)) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_wakeup" ::: "memory");
@@ -181,9 +202,11 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     const int pos = p0 + sw * kWave + lane;
     const bool valid = pos < p1;
     int row = 0, step = -1, p = 0, end = 0;
-    double diag = 1.0, bv = 0.0;
+    double diag = 1.0, bv[M];
     int sl[UNROLL];
     double v[UNROLL];
+#pragma unroll
+    for (int c = 0; c < M; ++c) bv[c] = 0.0;
     auto load_window = [&](int q) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -202,7 +225,9 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
         diag = a.bval[s];
         p = s + 1;
         end = a.brp[pos + 1];
-        bv = a.b[row];
+#pragma unroll
+        for (int c = 0; c < M; ++c)
+            if (M == 1 || c < a.m) bv[c] = a.b[(size_t)row * mm + c];
         load_window(p);
     }
     wait_vmem();
@@ -210,22 +235,36 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) nsteps = max(nsteps, __shfl_xor(nsteps, o));
     bool dead = false;
+    // one entry's values (M of them, a.m used) from its slot: in LDS, or (slot < 0) from x
+    auto read = [&](int u, U (&xv)[UNROLL][M], bool direct) {
+#pragma unroll
+        for (int c = 0; c < M; ++c)
+            xv[u][c] = (direct && sl[u] < 0) ? poll_bits(a.x + (size_t)(-sl[u] - 1) * mm + c)
+                                             : lds_get(&slot[c * (R + XM) + sl[u]]);
+    };
+    auto ready = [&](int u, int cnt, const U (&xv)[UNROLL][M]) {
+        bool ok = true;
+#pragma unroll
+        for (int c = 0; c < M; ++c) ok &= u >= cnt || (M > 1 && c >= a.m) || xv[u][c] != S;
+        return ok;
+    };
     for (int st = 0; st < nsteps && !dead; ++st) {
         const bool mine = step == st;
-        double acc = 0.0;
+        double acc[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) acc[c] = 0.0;
         if constexpr (SHORT) {
             // every row fits one window: read the step's values, wait in LDS for missing
-            // external ones, then the ordered sum
+            // external ones, then the ordered sums
             const int cnt = end - p;
-            U xv[UNROLL];
+            U xv[UNROLL][M];
             while (true) {
                 bool ok = true;
                 if (mine) {
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        xv[u] = lds_get(&slot[sl[u]]);
-                        ok &= u >= cnt || xv[u] != S;
-                    }
+                    for (int u = 0; u < UNROLL; ++u) read(u, xv, false);  // all reads first,
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) ok &= ready(u, cnt, xv);  // then the checks
                 }
                 if (__all(ok)) break;
                 __builtin_amdgcn_s_sleep(10);
@@ -236,10 +275,12 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
             }
             if (mine) {
 #pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    const double t2 = acc + v[u] * __builtin_bit_cast(double, xv[u]);
-                    acc = u < cnt ? t2 : acc;
-                }
+                for (int c = 0; c < M; ++c)
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) {
+                        const double t2 = acc[c] + v[u] * __builtin_bit_cast(double, xv[u][c]);
+                        acc[c] = u < cnt ? t2 : acc[c];
+                    }
             }
         } else {
             for (int q = p;; q += UNROLL) {
@@ -250,15 +291,14 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
                     wait_vmem();
                 }
                 const int cnt = min(UNROLL, end - q);
-                U xv[UNROLL];
+                U xv[UNROLL][M];
                 while (true) {
                     bool ok = true;
                     if (active) {
 #pragma unroll
-                        for (int u = 0; u < UNROLL; ++u) {
-                            xv[u] = sl[u] >= 0 ? lds_get(&slot[sl[u]]) : poll_bits(a.x + (-sl[u] - 1));
-                            ok &= u >= cnt || xv[u] != S;
-                        }
+                        for (int u = 0; u < UNROLL; ++u) read(u, xv, true);
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) ok &= ready(u, cnt, xv);
                     }
                     if (__all(ok)) break;
                     __builtin_amdgcn_s_sleep(10);
@@ -270,17 +310,24 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
                 if (dead) break;
                 if (active) {
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        const double t2 = acc + v[u] * __builtin_bit_cast(double, xv[u]);
-                        acc = u < cnt ? t2 : acc;
-                    }
+                    for (int c = 0; c < M; ++c)
+#pragma unroll
+                        for (int u = 0; u < UNROLL; ++u) {
+                            const double t2 = acc[c] + v[u] * __builtin_bit_cast(double, xv[u][c]);
+                            acc[c] = u < cnt ? t2 : acc[c];
+                        }
                 }
             }
         }
         if (mine && !dead) {
-            const double xi = (bv - acc) / diag;
-            lds_put(&slot[sw * kWave + lane], __builtin_bit_cast(U, xi));
-            publish(a.x + row, xi);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                if (M == 1 || c < a.m) {
+                    const double xi = (bv[c] - acc[c]) / diag;
+                    lds_put(&slot[c * (R + XM) + sw * kWave + lane], __builtin_bit_cast(U, xi));
+                    publish(a.x + (size_t)row * mm + c, xi);
+                }
+            }
         }
         // the block's other compute waves may sleep on this step's values
         if (W > 1 && __any(mine)) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_wakeup" ::: "memory");
@@ -456,6 +503,8 @@ int build_blocks(sptrsv_plan_s* P) {
     P->blk_kind = kind;
     P->blk_maxlen = 0;
     for (int r = 0; r < n; ++r) P->blk_maxlen = std::max(P->blk_maxlen, rp[r + 1] - rp[r] - 1);
+    P->blk_maxext = 0;
+    for (size_t b = 0; b + 1 < eptr.size(); ++b) P->blk_maxext = std::max(P->blk_maxext, eptr[b + 1] - eptr[b]);
     return SPTRSV_OK;
 }
 
@@ -472,28 +521,48 @@ void block_free(sptrsv_plan_s* p) {
     p->blk_maxlen = 0;
 }
 
+// Several right-hand sides per lane run on 512-row blocks of at most 4 entries per row and at
+// most 256 external rows per block (each LDS slot then holds M values); builds the block order
+// on first use.
+bool block_supports(sptrsv_plan_s* P, int m) {
+    if (m == 1) return true;
+    if (m < 1 || m > kBlockMaxM || P->n == 0) return false;
+    if (P->n_blocks == 0 && build_blocks(P) != SPTRSV_OK) return false;
+    return P->blk_rows == kBigRows && P->blk_maxlen <= 4 && P->blk_maxext <= 4 * kWave;
+}
+
 int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, double* d_x, int m, uint64_t limit) {
-    if (m != 1) return fail(SPTRSV_ERR_UNSUPPORTED, "block variant solves one right-hand side");
     if (precision != SPTRSV_FP64) return fail(SPTRSV_ERR_UNSUPPORTED, "block variant is FP64");
+    if (m < 1 || m > kBlockMaxM)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "block variant solves 1.." + std::to_string(kBlockMaxM) + " right-hand sides");
     if (P->n == 0) return SPTRSV_OK;
     if (P->n_blocks == 0) SPT_CHECK(build_blocks(P));
-    const int64_t n = P->n;
-    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
-    k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, n, P->ticket);
-    BlockArgs a{P->n_blocks, P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
+    const bool big = P->blk_rows == kBigRows;
+    if (!block_supports(P, m))
+        return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: several right-hand sides on grid factors only");
+    const int64_t nm = (int64_t)P->n * m;
+    const unsigned g = (unsigned)std::min<int64_t>((nm + 255) / 256, 65536);
+    k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, nm, P->ticket);
+    BlockArgs a{P->n_blocks, m, P->blk_ptr, P->blk_eptr, P->blk_ecol, P->blk_perm, P->blk_rp, P->blk_col, P->blk_step,
                 P->blk_val, d_b, d_x, P->ticket, limit};
     // resident workgroups per CU: dynamic LDS sized so that at most that many share a CU (160 KiB
     // of LDS per CU).  64-row blocks: kNB blocks per workgroup, `occ` blocks per CU (default
     // 4); 512-row blocks: one per workgroup, `occ` per CU (default 3)
-    const bool big = P->blk_rows == kBigRows;
     const int nb = big ? 1 : kNB;
+    const int M = m;  // one kernel instance per right-hand-side count
     const int per_cu = std::max(1, (occ > 0 ? std::min(occ, 32) : big ? 3 : 4) / nb);
-    const size_t own = (size_t)nb * (P->blk_rows + kMaxExt) * 8 + 512;
-    const size_t lds = per_cu >= 8 ? 0 : ((163840 / per_cu - own) & ~(size_t)255);
+    const size_t own = (size_t)nb * (P->blk_rows + (M == 1 ? kMaxExt : 4 * kWave)) * M * 8 + 512;
+    const size_t lds = per_cu >= 8 || own * per_cu >= 163840 ? 0 : ((163840 / per_cu - own) & ~(size_t)255);
     const unsigned grid = (unsigned)((P->n_blocks + nb - 1) / nb);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (big) {
+    if (m == 2) {
+        k_block<4, true, 1, 8, 2><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    } else if (m == 3) {
+        k_block<4, true, 1, 8, 3><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    } else if (m == 4) {
+        k_block<4, true, 1, 8, 4><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    } else if (big) {
         if (P->blk_maxlen <= 4) k_block<4, true, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
         else k_block<4, false, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
     } else {

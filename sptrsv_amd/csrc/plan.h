@@ -109,7 +109,8 @@ struct sptrsv_plan_s {
     int n_blocks = 0;
     int blk_kind = 0;
     int blk_maxlen = 0;          // most off-diagonal entries in a row
-    int blk_rows = 0;            // rows per block the layout was built for (64 or 256)
+    int blk_rows = 0;            // rows per block the layout was built for (64 or 512)
+    int blk_maxext = 0;          // longest external list of a block
     int* blk_ptr = nullptr;
     int* blk_eptr = nullptr;     // per block: its external values blk_ecol[blk_eptr[k] ..)
     int* blk_ecol = nullptr;
@@ -155,6 +156,7 @@ int tile_solve(sptrsv_plan_s* p, int precision, int rows, const double* d_b, dou
 void tile_free(sptrsv_plan_s* p);         // tile.hip
 int block_solve(sptrsv_plan_s* p, int precision, int occ, const double* d_b, double* d_x, int m,
                 uint64_t limit);  // block.hip
+bool block_supports(sptrsv_plan_s* p, int m);  // block.hip
 void block_free(sptrsv_plan_s* p);        // block.hip
 bool detect_grid(int n, const int* rowptr, const int* col, int64_t* dims);  // block.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip

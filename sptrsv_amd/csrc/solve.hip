@@ -678,10 +678,12 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 
 // AUTO: one right-hand side over a factor whose levels hold fewer than four rows on average
 // (a dependency chain) goes to the single-wave chain kernel, which keeps each window of rows in
-// LDS instead of paying a device-wide round trip per level; everything else to LEVEL_PULL.
-int default_variant(const sptrsv_plan_s* P, int m, int precision) {
+// LDS instead of paying a device-wide round trip per level; one to three right-hand sides in
+// FP64 over a natural-order grid factor to the block kernel (bricks solved in LDS; with four,
+// its per-step work outgrows the level-ordered pull kernel's); everything else to LEVEL_PULL.
+int default_variant(sptrsv_plan_s* P, int m, int precision) {
     if (m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
-    if (m == 1 && precision == SPTRSV_FP64 && P->grid) return SPTRSV_VAR_BLOCK;
+    if (m <= 3 && precision == SPTRSV_FP64 && P->grid && block_supports(P, m)) return SPTRSV_VAR_BLOCK;
     return SPTRSV_VAR_LEVEL_PULL;
 }
 

@@ -124,16 +124,40 @@ def test_chunks_of_a_general_factor():
     assert kind == 2 and sizes.max() <= 64 and np.median(sizes) == 64
 
 
+GRID_CASES = [k for k, (U, kind) in CASES.items() if k.startswith(("ic0", "grid", "wrap"))]
+
+
+@pytest.mark.parametrize("m", [2, 3, 4])
+@pytest.mark.parametrize("name", GRID_CASES)
+def test_block_several_rhs(name, m):
+    """2..4 right-hand sides per lane on grid factors (8x8x8 / 32x16 bricks, or the band order
+    of a grid whose bricks are not topological): x bitwise equal to the oracle, column by
+    column"""
+    U, kind = CASES[name]
+    banner(f"block m={m} {name}")
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, m, seed=7, layout="row")
+        x = P.solve_host(b, capi.params(variant=capi.VAR_BLOCK))
+        ref = np.ascontiguousarray(O.base_gauss_upper(U.rowptr, U.col, U.val, np.ascontiguousarray(b.T))[0].T)
+        assert np.array_equal(x, ref), (name, m, O.rel_error(ref, x))
+
+
 def test_block_rejects():
     U = CASES["ic0_12"][0]
     with capi.Plan.from_upper(U) as P:
-        xt, b = gen.rhs_for(U, 2, seed=1, layout="row")
+        xt, b = gen.rhs_for(U, 5, seed=1, layout="row")
         with pytest.raises(capi.SptrsvError) as e:
             P.solve_host(b, capi.params(variant=capi.VAR_BLOCK))
         assert e.value.code == capi.ERR_UNSUPPORTED
         xt, b = gen.rhs_for(U, 1, seed=1)
         with pytest.raises(capi.SptrsvError) as e:
             P.solve_host(b, capi.params(variant=capi.VAR_BLOCK, precision=capi.FP32))
+        assert e.value.code == capi.ERR_UNSUPPORTED
+    U = CASES["zipf_3000"][0]  # not a grid: one right-hand side only
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 2, seed=1, layout="row")
+        with pytest.raises(capi.SptrsvError) as e:
+            P.solve_host(b, capi.params(variant=capi.VAR_BLOCK))
         assert e.value.code == capi.ERR_UNSUPPORTED
 
 
@@ -143,12 +167,15 @@ def test_block_transposed():
     import scipy.sparse as sp
     from scipy.sparse.linalg import spsolve_triangular
     U = CASES["grid3d_9x6x5"][0]
-    with capi.Plan.from_upper(U) as P:
-        xt, b = gen.rhs_for(U, 1, seed=9)
-        got = P.solve_transposed_host(b, capi.params(variant=capi.VAR_BLOCK))
     A = sp.csr_matrix((U.val, U.col, U.rowptr), shape=(U.n, U.n))
-    ref = spsolve_triangular(A.T.tocsr(), b, lower=True)
-    assert O.rel_error(ref, got) <= 1e-12
+    with capi.Plan.from_upper(U) as P:
+        for m in (1, 3):
+            xt, b = gen.rhs_for(U, m, seed=9, layout="row")
+            if m == 1:
+                b = b.reshape(-1)
+            got = P.solve_transposed_host(b, capi.params(variant=capi.VAR_BLOCK))
+            ref = spsolve_triangular(A.T.tocsr(), b, lower=True)
+            assert O.rel_error(ref, got) <= 1e-12
 
 
 @pytest.mark.parametrize("name", ["chain_4000", "grid3d_8x8x1600"])

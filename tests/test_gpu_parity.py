@@ -299,16 +299,19 @@ def test_ccs2ccs_pad_device_rejects_bad_partition(plans):
 
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
-    m == 1, natural-order grid factors to the block kernel at m == 1 in FP64, and everything
+    m == 1, natural-order grid factors to the block kernel at m <= 3 in FP64, and everything
     else to the level-ordered pull kernel (the last two bitwise exact)."""
     assert plans["ic0_12"].auto_variant(1) == capi.VAR_BLOCK
+    assert plans["ic0_12"].auto_variant(2) == capi.VAR_BLOCK
     assert plans["ic0_12"].auto_variant(1, capi.FP32) == capi.VAR_LEVEL_PULL
-    assert plans["ic0_12"].auto_variant(3) == capi.VAR_LEVEL_PULL
+    assert plans["ic0_12"].auto_variant(3) == capi.VAR_BLOCK
+    assert plans["ic0_12"].auto_variant(4) == capi.VAR_LEVEL_PULL
     assert plans["zipf_3000"].auto_variant(1) in (capi.VAR_CHAIN, capi.VAR_LEVEL_PULL)
+    assert plans["zipf_3000"].auto_variant(2) == capi.VAR_LEVEL_PULL
     assert plans["chol2d_15"].auto_variant(1) != capi.VAR_BLOCK
     for name in MATS:
         U, P = MATS[name], plans[name]
-        for m in (1, 3):
+        for m in (1, 2, 3, 4):
             xt, b = gen.rhs_for(U, m, seed=22, layout="row")
             ref = oracle_solve(U, b)
             x = P.solve_host(b, capi.params())

@@ -521,6 +521,16 @@ void block_free(sptrsv_plan_s* p) {
     p->blk_maxlen = 0;
 }
 
+// 512-row blocks (8 compute waves + a poller) whose rows all fit one UNROLL-entry window, m
+// right-hand sides per lane
+template <int UNROLL>
+void launch_bricks(const BlockArgs& a, int m, unsigned grid, size_t lds, hipStream_t s) {
+    if (m == 1) k_block<UNROLL, true, 1, 8, 1><<<grid, 9 * kWave, lds, s>>>(a);
+    else if (m == 2) k_block<UNROLL, true, 1, 8, 2><<<grid, 9 * kWave, lds, s>>>(a);
+    else if (m == 3) k_block<UNROLL, true, 1, 8, 3><<<grid, 9 * kWave, lds, s>>>(a);
+    else k_block<UNROLL, true, 1, 8, 4><<<grid, 9 * kWave, lds, s>>>(a);
+}
+
 // Several right-hand sides per lane run on 512-row blocks of at most 4 entries per row and at
 // most 256 external rows per block (each LDS slot then holds M values); builds the block order
 // on first use.
@@ -556,15 +566,12 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     const unsigned grid = (unsigned)((P->n_blocks + nb - 1) / nb);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (m == 2) {
-        k_block<4, true, 1, 8, 2><<<grid, 9 * kWave, lds, P->stream>>>(a);
-    } else if (m == 3) {
-        k_block<4, true, 1, 8, 3><<<grid, 9 * kWave, lds, P->stream>>>(a);
-    } else if (m == 4) {
-        k_block<4, true, 1, 8, 4><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    if (big && P->blk_maxlen <= 3) {  // 3D grids: at most 3 off-diagonals, one 3-entry window
+        launch_bricks<3>(a, m, grid, lds, P->stream);
+    } else if (big && P->blk_maxlen <= 4) {
+        launch_bricks<4>(a, m, grid, lds, P->stream);
     } else if (big) {
-        if (P->blk_maxlen <= 4) k_block<4, true, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
-        else k_block<4, false, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
+        k_block<4, false, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
     } else {
         if (P->blk_maxlen <= 4) k_block<4, true, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
         else k_block<4, false, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);

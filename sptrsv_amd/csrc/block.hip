@@ -521,24 +521,23 @@ void block_free(sptrsv_plan_s* p) {
     p->blk_maxlen = 0;
 }
 
-// 512-row blocks (8 compute waves + a poller) whose rows all fit one UNROLL-entry window, m
-// right-hand sides per lane
-template <int UNROLL>
+// 512-row blocks (8 compute waves + a poller), m right-hand sides per lane.  They are built
+// for grid factors only, whose rows hold at most 3 off-diagonals (three distinct offsets): one
+// 3-entry window per row.
 void launch_bricks(const BlockArgs& a, int m, unsigned grid, size_t lds, hipStream_t s) {
-    if (m == 1) k_block<UNROLL, true, 1, 8, 1><<<grid, 9 * kWave, lds, s>>>(a);
-    else if (m == 2) k_block<UNROLL, true, 1, 8, 2><<<grid, 9 * kWave, lds, s>>>(a);
-    else if (m == 3) k_block<UNROLL, true, 1, 8, 3><<<grid, 9 * kWave, lds, s>>>(a);
-    else k_block<UNROLL, true, 1, 8, 4><<<grid, 9 * kWave, lds, s>>>(a);
+    if (m == 1) k_block<3, true, 1, 8, 1><<<grid, 9 * kWave, lds, s>>>(a);
+    else if (m == 2) k_block<3, true, 1, 8, 2><<<grid, 9 * kWave, lds, s>>>(a);
+    else if (m == 3) k_block<3, true, 1, 8, 3><<<grid, 9 * kWave, lds, s>>>(a);
+    else k_block<3, true, 1, 8, 4><<<grid, 9 * kWave, lds, s>>>(a);
 }
 
-// Several right-hand sides per lane run on 512-row blocks of at most 4 entries per row and at
-// most 256 external rows per block (each LDS slot then holds M values); builds the block order
-// on first use.
+// Several right-hand sides per lane run on 512-row blocks with at most 256 external rows per
+// block (each LDS slot then holds M values); builds the block order on first use.
 bool block_supports(sptrsv_plan_s* P, int m) {
     if (m == 1) return true;
     if (m < 1 || m > kBlockMaxM || P->n == 0) return false;
     if (P->n_blocks == 0 && build_blocks(P) != SPTRSV_OK) return false;
-    return P->blk_rows == kBigRows && P->blk_maxlen <= 4 && P->blk_maxext <= 4 * kWave;
+    return P->blk_rows == kBigRows && P->blk_maxlen <= 3 && P->blk_maxext <= 4 * kWave;
 }
 
 int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, double* d_x, int m, uint64_t limit) {
@@ -548,6 +547,8 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     if (P->n == 0) return SPTRSV_OK;
     if (P->n_blocks == 0) SPT_CHECK(build_blocks(P));
     const bool big = P->blk_rows == kBigRows;
+    if (big && P->blk_maxlen > 3)  // not built: grid factors have three distinct offsets at most
+        return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: 512-row blocks with rows of more than 3 off-diagonals");
     if (!block_supports(P, m))
         return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: several right-hand sides on grid factors only");
     const int64_t nm = (int64_t)P->n * m;
@@ -566,12 +567,8 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     const unsigned grid = (unsigned)((P->n_blocks + nb - 1) / nb);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    if (big && P->blk_maxlen <= 3) {  // 3D grids: at most 3 off-diagonals, one 3-entry window
-        launch_bricks<3>(a, m, grid, lds, P->stream);
-    } else if (big && P->blk_maxlen <= 4) {
-        launch_bricks<4>(a, m, grid, lds, P->stream);
-    } else if (big) {
-        k_block<4, false, 1, 8><<<grid, 9 * kWave, lds, P->stream>>>(a);
+    if (big) {
+        launch_bricks(a, m, grid, lds, P->stream);
     } else {
         if (P->blk_maxlen <= 4) k_block<4, true, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);
         else k_block<4, false, kNB, 1><<<grid, 2 * kNB * kWave, lds, P->stream>>>(a);

@@ -159,7 +159,9 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
 
 /* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
  * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average, BLOCK for FP64
- * m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise. */
+ * m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise.  For 2 <= m <= 3 on a grid
+ * factor the choice checks the block layout's limits and so builds that layout (on the host,
+ * once per plan; the block solve would build it on its first call). */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
 
 /* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).

@@ -550,7 +550,9 @@ int block_solve(sptrsv_plan_s* P, int precision, int occ, const double* d_b, dou
     if (big && P->blk_maxlen > 3)  // not built: grid factors have three distinct offsets at most
         return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: 512-row blocks with rows of more than 3 off-diagonals");
     if (!block_supports(P, m))
-        return fail(SPTRSV_ERR_UNSUPPORTED, "block variant: several right-hand sides on grid factors only");
+        return fail(SPTRSV_ERR_UNSUPPORTED,
+                    "block variant: several right-hand sides need the 512-row blocks of a grid factor with at most " +
+                        std::to_string(4 * kWave) + " external rows per block");
     const int64_t nm = (int64_t)P->n * m;
     const unsigned g = (unsigned)std::min<int64_t>((nm + 255) / 256, 65536);
     k_fill_sentinel_b<<<g, 256, 0, P->stream>>>(d_x, nm, P->ticket);

@@ -142,6 +142,31 @@ def test_block_several_rhs(name, m):
         assert np.array_equal(x, ref), (name, m, O.rel_error(ref, x))
 
 
+@pytest.mark.parametrize("m", [2, 3, 4])
+def test_block_several_rhs_unaligned_x(m):
+    """x at an address 8 bytes past a 16-byte boundary (any 8-byte aligned x is valid): x
+    bitwise equal to the oracle"""
+    import ctypes
+    U = CASES["grid3d_9x6x5"][0]
+    banner(f"block m={m} unaligned x")
+    xt, b = gen.rhs_for(U, m, seed=8, layout="row")
+    ref = np.ascontiguousarray(O.base_gauss_upper(U.rowptr, U.col, U.val, np.ascontiguousarray(b.T))[0].T)
+    out = np.empty((U.n, m))
+    with capi.Plan.from_upper(U) as P:
+        d_b = capi.DeviceArray.from_host(np.ascontiguousarray(b))
+        big = capi.DeviceArray((U.n * m + 2,))  # device allocations are 256-byte aligned
+        try:
+            xp = ctypes.c_void_p(big.ptr.value + 8)
+            prm = capi.params(variant=capi.VAR_BLOCK)
+            capi.check(capi.lib().sptrsv_solve(P.handle, ctypes.byref(prm), d_b.ptr, xp, m), "solve")
+            P.sync()
+            capi.check(capi.lib().sptrsv_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), xp, out.nbytes), "d2h")
+        finally:
+            d_b.free()
+            big.free()
+    assert np.array_equal(out, ref), (m, O.rel_error(ref, out))
+
+
 def test_block_rejects():
     U = CASES["ic0_12"][0]
     with capi.Plan.from_upper(U) as P:

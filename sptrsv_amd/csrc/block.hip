@@ -79,6 +79,7 @@ constexpr int kBand = 16;                      // levels per band of the generic
 constexpr int kNB = 4;                         // 64-row blocks per workgroup (one ticket each)
 constexpr int kBigRows = 8 * kWave;            // rows of a grid factor's block: 8 compute waves
 constexpr int kBlockMaxM = 4;                  // right-hand sides per lane (grid factors)
+constexpr int kPolled = 1 << 30;               // step word flag: another block reads this row
 
 struct BlockArgs {
     int nblk;
@@ -90,7 +91,7 @@ struct BlockArgs {
     const int* __restrict__ brp;    // CSR in position order (diagonal first)
     const int* __restrict__ bslot;  // LDS slot of each entry's x value (< 0: -(column + 1), read
                                     // from x: entries of a row beyond the block's kMaxExt)
-    const int* __restrict__ bstep;  // step of each position inside its block
+    const int* __restrict__ bstep;  // step of each position inside its block (| kPolled)
     const double* __restrict__ bval;
     const double* __restrict__ b;
     double* x;
@@ -202,6 +203,8 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     const int pos = p0 + sw * kWave + lane;
     const bool valid = pos < p1;
     int row = 0, step = -1, p = 0, end = 0;
+    bool polled = false;  // another block reads this row from x: publish it at its step
+    double xk[M];         // otherwise its values, written once the wave's steps are done
     double diag = 1.0, bv[M];
     int sl[UNROLL];
     double v[UNROLL];
@@ -220,7 +223,8 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
     };
     if (valid) {
         row = a.bperm[pos];
-        step = a.bstep[pos];
+        step = a.bstep[pos] & (kPolled - 1);
+        polled = a.bstep[pos] >= kPolled;
         const int s = a.brp[pos];
         diag = a.bval[s];
         p = s + 1;
@@ -325,12 +329,20 @@ __global__ __launch_bounds__((W + 1) * NB * kWave) void k_block(BlockArgs a) {
                 if (M == 1 || c < a.m) {
                     const double xi = (bv[c] - acc[c]) / diag;
                     lds_put(&slot[c * (R + XM) + sw * kWave + lane], __builtin_bit_cast(U, xi));
-                    publish(a.x + (size_t)row * mm + c, xi);
+                    if (polled) publish(a.x + (size_t)row * mm + c, xi);
+                    xk[c] = xi;
                 }
             }
         }
         // the block's other compute waves may sleep on this step's values
         if (W > 1 && __any(mine)) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_wakeup" ::: "memory");
+    }
+    // rows no other block reads leave the dependency chain: their scattered stores come after
+    // the wave's last step instead of inside every step
+    if (valid && !polled && !dead) {
+#pragma unroll
+        for (int c = 0; c < M; ++c)
+            if (M == 1 || c < a.m) publish(a.x + (size_t)row * mm + c, xk[c]);
     }
 }
 
@@ -489,6 +501,16 @@ int build_blocks(sptrsv_plan_s* P) {
     brp[(size_t)n] = (int)e0;
     bptr.push_back(n);
     eptr.push_back((int)ecol.size());
+    // rows another block reads from x (a poller's list or a direct read) are published at their
+    // step; the rest once their wave is done (steps stay far below kPolled)
+    {
+        std::vector<char> read_by_other((size_t)n, 0);
+        for (int c : ecol) read_by_other[(size_t)c] = 1;
+        for (int64_t e = 0; e < nnz; ++e)
+            if (bslot[(size_t)e] < 0) read_by_other[(size_t)(-bslot[(size_t)e] - 1)] = 1;
+        for (int pos = 0; pos < n; ++pos)
+            if (read_by_other[(size_t)order[(size_t)pos]]) bstep[(size_t)pos] |= kPolled;
+    }
     ecol.resize(ecol.size() + kMaxExt, 0);  // the poller loads kMaxExt entries from every block's start
     block_free(P);
     SPT_CHECK(upload_b(&P->blk_ptr, bptr));

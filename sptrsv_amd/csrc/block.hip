@@ -11,7 +11,9 @@
 //     block's own rows from their slots as soon as their lanes store them, the values of
 //     earlier blocks from slots the poller fills; it never waits on global memory;
 //   * the poller wave holds the block's list of external values (at most kMaxExt) and polls
-//     the self-publishing x for them until all have arrived, copying each into its LDS slot.
+//     the self-publishing x for them until all have arrived, copying each into its LDS slot;
+//   * a row's value goes to x at its step only if another block reads it; the others are
+//     stored after the wave's last step, off the dependency chain.
 // Grid factors use 512-row blocks (8 compute waves), other factors 64-row blocks, four of them
 // per workgroup under one ticket.
 // A dependency inside the block costs an LDS round trip; only a dependency on another block

@@ -238,7 +238,13 @@ int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t
                         const double* val, sptrsv_batch_t* batch);
 int sptrsv_batch_destroy(sptrsv_batch_t batch);
 int sptrsv_batch_info(sptrsv_batch_t batch, int* nsys, int* n, int64_t* nnz);
-/* 24 selector features per system (row-major nsys x 24), see DESIGN.md for the list. */
+/* 24 selector features per system (row-major nsys x 24), computed on the host when the batch
+ * is created: log2 n, log2 nnz, row length mean / max / std, levels per row, log2 levels,
+ * log2 mean level width, max level width / n, level width std / mean, diagonal-only rows / n,
+ * mean and max column offset (relative to n), fractions of offsets equal to 1, at most 64 and
+ * above 4096, rows with an entry in the next column / n, supernodes / n, mean and max
+ * supernode width, rows in supernodes of width >= 16 / n, mean largest offset per row / n^2,
+ * log2 m, log2 of the mean |diagonal| over the mean |off-diagonal|. */
 int sptrsv_batch_features(sptrsv_batch_t batch, int m, float* feats);
 /* Per-system configuration and processing order (either may be NULL). */
 int sptrsv_batch_set_configs(sptrsv_batch_t batch, const int* cfg, const int* order);

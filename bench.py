@@ -272,7 +272,7 @@ def main():
     t_start = time.perf_counter()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the solve path here is single-GPU "
-              "(multi-GPU sharding is design-only for this build)", file=sys.stderr)
+              "(multi-GPU sharding is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

@@ -13,7 +13,7 @@ drop-in entry point of that name runs:
     barrier      barrier_upper             LEVELSET (one launch per level)
     syncfree     sptrsv_syncfree_opencl    AUTO (BLOCK on natural-order grid factors)
     write_first  sptrsv_syncfree3_opencl   ROW_PULL
-    block / level_pull / tile / chain / csc_push    this build's other kernels
+    block / level_pull / tile / chain / csc_push    the library's other kernels
 
 A process prints "Time: <seconds>" last on stderr, like the reference CLI (src/utils.cpp:353),
 where the time is the device solve time of one solve with data resident on the GPU (the

@@ -473,7 +473,7 @@ def test_cpp_dropin_entry_points(tmp_path, which, rhs):
     assert out.returncode == 0, out.stdout + out.stderr
     n = U.n
     cnt = np.arange(n * rhs)
-    b = (1.0 + (cnt * 7) % 11).reshape(rhs, n)  # column-major as written by the driver
+    b = (1.0 + (cnt * 7) % 11).reshape(rhs, n)  # column-major as written by the C++ test program
     ref, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
     load = lambda k: np.fromfile(str(tmp_path / f"{k}.bin"), dtype=np.float64)
     for k in ("base", "base_new", "barrier"):
@@ -613,7 +613,8 @@ def test_bf16_supernodal_refined_reaches_fp64(m):
 
 def test_refinement_reports_non_convergence():
     """diag_scale << 1 removes the diagonal dominance: FP32 solves cannot contract the error,
-    and the driver must say so through the returned estimate instead of claiming convergence."""
+    and the refinement loop must say so through the returned estimate instead of claiming
+    convergence."""
     U = gen.random_zipf(2000, seed=12, diag_scale=0.02)
     with capi.Plan.from_upper(U) as P:
         xt, b = gen.rhs_for(U, 1, seed=1)

@@ -85,7 +85,9 @@ typedef struct {
     int block;        /* pull kernels: threads per workgroup (64..1024, power of two); 0 =
                          default: LEVEL_PULL 64 rows per workgroup, ROW_PULL 256 / 1024 threads.
                          TILE: rows per tile (1..16384; 0 = 16384) */
-    int reserved[1];
+    int exact;        /* AUTO only: nonzero restricts the choice to kernels whose FP64 x is
+                         bitwise equal to the serial solver's (BLOCK, LEVEL_PULL): the drop-in
+                         replacements of base_gauss_upper use it */
 } sptrsv_params_t;
 
 typedef struct sptrsv_plan_s* sptrsv_plan_t;
@@ -158,8 +160,8 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
                         double* val /* nnz */);
 
 /* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
- * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average, BLOCK for FP64
- * m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise.  For 2 <= m <= 3 on a grid
+ * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average (never with
+ * params.exact), BLOCK for FP64 m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise.  For 2 <= m <= 3 on a grid
  * factor the choice checks the block layout's limits and so builds that layout (on the host,
  * once per plan; the block solve would build it on its first call). */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
@@ -217,9 +219,38 @@ typedef struct {
     int max_iters;    /* 0 -> 30 */
     int precision;    /* SPTRSV_FP32 or SPTRSV_BF16 */
     double tol;       /* 0 -> 1e-13 (estimated relative error of x) */
+    int fallback;     /* nonzero: when the iteration ends without meeting tol, x is recomputed
+                         by the FP64 solve (AUTO variant, or SUPERNODAL for a supernodal one) */
+    int reserved;
 } sptrsv_refine_t;
 int sptrsv_solve_refined(sptrsv_plan_t plan, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
                          const double* d_b, double* d_x, int m, int* iters, double* ratio);
+
+/* How a refined solve ended (no reference counterpart). */
+#define SPTRSV_REFINE_CONVERGED 0  /* estimate <= tol */
+#define SPTRSV_REFINE_FLOOR 1      /* corrections stopped shrinking by half (FP64 residual floor,
+                                      or a contraction too slow for the inner precision) */
+#define SPTRSV_REFINE_MAX_ITERS 2  /* max_iters corrections applied, estimate still > tol */
+#define SPTRSV_REFINE_DIVERGED 3   /* a correction or x not finite, or corrections growing:
+                                      the inner solve cannot resolve this factor (kappa * u >= 1) */
+typedef struct {
+    int iters;        /* corrections applied */
+    int outcome;      /* SPTRSV_REFINE_* of the reduced-precision iteration */
+    int fell_back;    /* 1: x comes from the FP64 fallback solve */
+    int variant;      /* kernel family of the reduced-precision solves */
+    double estimate;  /* error estimate of the last reduced-precision iterate (+inf when a norm
+                         was not finite) */
+} sptrsv_refine_info_t;
+/* sptrsv_solve_refined with the outcome reported (info may be NULL). */
+int sptrsv_solve_refined_ex(sptrsv_plan_t plan, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
+                            const double* d_b, double* d_x, int m, sptrsv_refine_info_t* info);
+
+/* 1-norm condition estimate kappa_1(U) = ||U||_1 * est(||U^-1||_1) (no reference counterpart):
+ * Hager's method as refined by Higham (LAPACK xLACON's iteration: at most 5 pairs of FP64
+ * solves with U and U^T on the device, plus the alternating-sign test vector).  Synchronous.
+ * *norm_inv receives the estimate of ||U^-1||_1 (a lower bound, usually exact or within a
+ * small factor); either output may be NULL. */
+int sptrsv_condest(sptrsv_plan_t plan, double* kappa, double* norm_inv);
 
 /* ---- batches of independent systems (no reference counterpart) -------------------------
  * The systems are given as one block-diagonal upper CSR: system s owns rows
@@ -255,6 +286,8 @@ int sptrsv_batch_sync(sptrsv_batch_t batch);
  * Synchronous; *estimate = estimated relative error of x. */
 int sptrsv_batch_solve_refined(sptrsv_batch_t batch, const sptrsv_refine_t* opts, const double* d_b, double* d_x,
                                int m, int* iters, double* estimate);
+int sptrsv_batch_solve_refined_ex(sptrsv_batch_t batch, const sptrsv_refine_t* opts, const double* d_b,
+                                  double* d_x, int m, sptrsv_refine_info_t* info);
 /* The block-diagonal plan under the batch (for residuals, timing, refinement). */
 sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t batch);
 /* Autotuner primitive: every system on `cfg` in one launch; device microseconds per system. */

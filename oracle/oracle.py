@@ -97,6 +97,29 @@ def base_gauss_upper(rowptr, col, val, b, new=False):
     return (x.reshape(np.shape(b)), t)
 
 
+def base_gauss_upper_columns(rowptr, col, val, b, workers=16):
+    """base_gauss_upper over the columns of b (m, n) split across host threads.  The serial
+    solve's RHS loop is outermost (src/core.cpp:21-30), so every column is computed exactly as in
+    one call: the result is bitwise that of base_gauss_upper.  ctypes releases the GIL."""
+    from concurrent.futures import ThreadPoolExecutor
+    rp, c, v = _csr(rowptr, col, val)
+    n = len(rp) - 1
+    bb = _rhs(b, n)
+    m = bb.shape[0]
+    x = np.zeros_like(bb)
+    w = max(1, min(workers, m))
+    cuts = np.linspace(0, m, w + 1).astype(int)
+
+    def part(k):
+        k0, k1 = cuts[k], cuts[k + 1]
+        if k1 > k0:
+            lib().orc_base_gauss_upper(n, v, rp, c, x[k0:k1], np.ascontiguousarray(bb[k0:k1]), int(k1 - k0))
+
+    with ThreadPoolExecutor(w) as ex:
+        list(ex.map(part, range(w)))
+    return x.reshape(np.shape(b))
+
+
 def base_gauss_lower(rowptr, col, val, b):
     rp, c, v = _csr(rowptr, col, val)
     n = len(rp) - 1

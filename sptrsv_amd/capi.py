@@ -46,15 +46,28 @@ class SptrsvError(RuntimeError):
         self.code = code
 
 
+REFINE_CONVERGED, REFINE_FLOOR, REFINE_MAX_ITERS, REFINE_DIVERGED = range(4)
+REFINE_OUTCOMES = ["converged", "floor", "max_iters", "diverged"]
+
+
 class RefineOpts(ctypes.Structure):
-    _fields_ = [("max_iters", ctypes.c_int), ("precision", ctypes.c_int), ("tol", ctypes.c_double)]
+    _fields_ = [("max_iters", ctypes.c_int), ("precision", ctypes.c_int), ("tol", ctypes.c_double),
+                ("fallback", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class RefineInfo(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("outcome", ctypes.c_int), ("fell_back", ctypes.c_int),
+                ("variant", ctypes.c_int), ("estimate", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"iters": self.iters, "outcome": REFINE_OUTCOMES[self.outcome], "fell_back": bool(self.fell_back),
+                "variant": self.variant, "estimate": self.estimate}
 
 
 class Params(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("group", ctypes.c_int), ("unroll", ctypes.c_int),
                 ("precision", ctypes.c_int), ("watchdog_us", ctypes.c_int),
-                ("occupancy", ctypes.c_int), ("block", ctypes.c_int),
-                ("reserved", ctypes.c_int * 1)]
+                ("occupancy", ctypes.c_int), ("block", ctypes.c_int), ("exact", ctypes.c_int)]
 
 
 _lib = None
@@ -135,6 +148,11 @@ def lib():
         "sptrsv_solve_refined": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                       ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_plan_get_supernodes": (_i, [_PLAN, ctypes.POINTER(_i), _vp]),
+        "sptrsv_solve_refined_ex": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
+                                         ctypes.POINTER(RefineInfo)]),
+        "sptrsv_batch_solve_refined_ex": (_i, [_vp, ctypes.POINTER(RefineOpts), _vp, _vp, _i,
+                                               ctypes.POINTER(RefineInfo)]),
+        "sptrsv_condest": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -149,11 +167,18 @@ def check(rc: int, where: str):
         raise SptrsvError(rc, where, lib().sptrsv_last_error().decode())
 
 
-def params(variant=VAR_AUTO, group=0, unroll=0, precision=FP64, watchdog_us=0, occupancy=0, block=0) -> Params:
+def params(variant=VAR_AUTO, group=0, unroll=0, precision=FP64, watchdog_us=0, occupancy=0, block=0,
+           exact=False) -> Params:
     p = Params()
     p.variant, p.group, p.unroll, p.precision, p.watchdog_us = variant, group, unroll, precision, watchdog_us
-    p.occupancy, p.block = occupancy, block
+    p.occupancy, p.block, p.exact = occupancy, block, int(bool(exact))
     return p
+
+
+def refine_opts(precision=FP32, tol=0.0, max_iters=0, fallback=False) -> RefineOpts:
+    o = RefineOpts()
+    o.max_iters, o.precision, o.tol, o.fallback = max_iters, precision, tol, int(bool(fallback))
+    return o
 
 
 def device_info():
@@ -409,12 +434,26 @@ class Plan:
     def solve_refined(self, d_b, d_x, m=1, prm=None, precision=FP32, tol=0.0, max_iters=0):
         """Reduced-precision solves + FP64 refinement on the device (synchronous).
         Returns (corrections applied, estimated relative error of x; converged when <= tol)."""
-        o = RefineOpts()
-        o.max_iters, o.precision, o.tol = max_iters, precision, tol
+        o = refine_opts(precision, tol, max_iters)
         it, ratio = _i(0), ctypes.c_double(0.0)
         check(lib().sptrsv_solve_refined(self.handle, ctypes.byref(prm or params()), ctypes.byref(o), d_b.ptr,
                                          d_x.ptr, int(m), ctypes.byref(it), ctypes.byref(ratio)), "solve_refined")
         return it.value, ratio.value
+
+    def solve_refined_ex(self, d_b, d_x, m=1, prm=None, precision=FP32, tol=0.0, max_iters=0, fallback=False):
+        """As solve_refined, returning the outcome report (dict: iters, outcome, fell_back,
+        variant, estimate); with fallback a run that does not converge ends with the FP64 solve."""
+        o = refine_opts(precision, tol, max_iters, fallback)
+        info = RefineInfo()
+        check(lib().sptrsv_solve_refined_ex(self.handle, ctypes.byref(prm or params()), ctypes.byref(o), d_b.ptr,
+                                            d_x.ptr, int(m), ctypes.byref(info)), "solve_refined_ex")
+        return info.as_dict()
+
+    def condest(self):
+        """(kappa_1(U) estimate, estimate of ||U^-1||_1) from FP64 solves on the device."""
+        k, ni = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(lib().sptrsv_condest(self.handle, ctypes.byref(k), ctypes.byref(ni)), "condest")
+        return k.value, ni.value
 
     def solve_transposed(self, d_b: DeviceArray, d_x: DeviceArray, m: int = 1, prm: Params | None = None):
         """Asynchronous device solve of U^T X = B (forward substitution, row-major n x m)."""
@@ -529,12 +568,19 @@ class Batch:
 
     def solve_refined(self, d_b, d_x, m=1, tol=0.0, max_iters=0):
         """FP32 batched solves + FP64 refinement (synchronous): (corrections, error estimate)."""
-        o = RefineOpts()
-        o.max_iters, o.precision, o.tol = max_iters, FP32, tol
+        o = refine_opts(FP32, tol, max_iters)
         it, est = _i(0), ctypes.c_double(0.0)
         check(lib().sptrsv_batch_solve_refined(self.handle, ctypes.byref(o), d_b.ptr, d_x.ptr, int(m), ctypes.byref(it),
                                                ctypes.byref(est)), "batch_solve_refined")
         return it.value, est.value
+
+    def solve_refined_ex(self, d_b, d_x, m=1, tol=0.0, max_iters=0, fallback=False):
+        """As solve_refined with the outcome report (see Plan.solve_refined_ex)."""
+        o = refine_opts(FP32, tol, max_iters, fallback)
+        info = RefineInfo()
+        check(lib().sptrsv_batch_solve_refined_ex(self.handle, ctypes.byref(o), d_b.ptr, d_x.ptr, int(m),
+                                                  ctypes.byref(info)), "batch_solve_refined_ex")
+        return info.as_dict()
 
     def time_config(self, cfg, d_b, d_x, m=1, precision=FP64):
         us = np.zeros(self.nsys, dtype=np.float64)

@@ -553,9 +553,10 @@ int sptrsv_batch_solve(sptrsv_batch_t B, int precision, const double* d_b, doubl
 }
 
 // FP32 batched solves with FP64 refinement: the residual runs over the whole block-diagonal
-// CSR, the stopping rule over the inf-norms of all systems together.  Synchronous.
-int sptrsv_batch_solve_refined(sptrsv_batch_t B, const sptrsv_refine_t* opts, const double* d_b, double* d_x, int m,
-                               int* iters, double* estimate) {
+// CSR, the stopping rule over the inf-norms of all systems together; with opts.fallback a run
+// that does not converge is finished by the FP64 batched solve.  Synchronous.
+int sptrsv_batch_solve_refined_ex(sptrsv_batch_t B, const sptrsv_refine_t* opts, const double* d_b, double* d_x,
+                                  int m, sptrsv_refine_info_t* info) {
     if (!B || !d_b || !d_x || m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "bad batch solve arguments");
     sptrsv_refine_t ro{};
     if (opts) ro = *opts;
@@ -567,7 +568,31 @@ int sptrsv_batch_solve_refined(sptrsv_batch_t B, const sptrsv_refine_t* opts, co
         return launch_batch<float>(B, P->val32, in, out, m, nullptr);
     };
     auto residual = [&](const double* b, const double* x, float* r) -> int { return residual32(P, b, x, r, m); };
-    return refine_loop(P, ro, d_b, d_x, m, solve_low, residual, iters, estimate);
+    RefineResult res{};
+    SPT_CHECK(refine_loop(P, ro, d_b, d_x, m, solve_low, residual, &res));
+    int fell_back = 0;
+    if (res.outcome != SPTRSV_REFINE_CONVERGED && ro.fallback) {
+        SPT_CHECK(launch_batch<double>(B, P->val, d_b, d_x, m, nullptr));
+        SPT_CHECK(sptrsv_plan_sync(P));
+        fell_back = 1;
+    }
+    if (info) {
+        info->iters = res.iters;
+        info->outcome = res.outcome;
+        info->fell_back = fell_back;
+        info->variant = -1;
+        info->estimate = res.estimate;
+    }
+    return SPTRSV_OK;
+}
+
+int sptrsv_batch_solve_refined(sptrsv_batch_t B, const sptrsv_refine_t* opts, const double* d_b, double* d_x, int m,
+                               int* iters, double* estimate) {
+    sptrsv_refine_info_t info{};
+    SPT_CHECK(sptrsv_batch_solve_refined_ex(B, opts, d_b, d_x, m, &info));
+    if (iters) *iters = info.iters;
+    if (estimate) *estimate = info.estimate;
+    return SPTRSV_OK;
 }
 
 int sptrsv_batch_sync(sptrsv_batch_t B) {

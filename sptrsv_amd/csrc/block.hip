@@ -370,12 +370,11 @@ void block_order(int n, const std::vector<int>& rp, const std::vector<int>& col,
     int64_t d[3];
     order.assign((size_t)n, 0);
     bptr.clear();
-    if (detect_grid(n, rp.data(), col.data(), d)) {
-        // 3D: 8x8x8 (512 rows) or 4x4x4 (64); 2D: 32x16 or 8x8
-        const bool big = R >= kBigRows;
-        const int64_t bx = d[2] > 1 ? (big ? 8 : 4) : (big ? 32 : 8);
-        const int64_t by = d[2] > 1 ? bx : (big ? 16 : 8);
-        const int64_t bz = d[2] > 1 ? bx : 1;
+    if (R == kBigRows && detect_grid(n, rp.data(), col.data(), d)) {
+        // bricks of 512 rows: 8x8x8 in 3D, 32x16 in 2D
+        const int64_t bx = d[2] > 1 ? 8 : 32;
+        const int64_t by = d[2] > 1 ? 8 : 16;
+        const int64_t bz = d[2] > 1 ? 8 : 1;
         const int64_t gx = (d[0] + bx - 1) / bx, gy = (d[1] + by - 1) / by, gz = (d[2] + bz - 1) / bz;
         // bricks by descending anti-diagonal (a brick's dependencies lie in bricks with larger
         // coordinates), then descending brick index; rows inside a brick by level, then

@@ -27,8 +27,9 @@ void report(const char* where, int rc) {
 
 // One GPU solve of an upper CSR (diagonal first).  Returns device solve seconds or the
 // negative status.
+// exact != 0: AUTO picks only kernels whose x is bitwise equal to the serial solver's.
 double gpu_upper_csr(const char* where, int n, const uint64_t* rowptr, const int* col, const double* val,
-                     double* x, const double* b, int rhs, int col_major, int variant) {
+                     double* x, const double* b, int rhs, int col_major, int variant, int exact = 0) {
     std::vector<int64_t> rp(rowptr, rowptr + n + 1);
     sptrsv_plan_t plan = nullptr;
     int rc = sptrsv_plan_create_csr(n, rp[n], rp.data(), col, val, &plan);
@@ -38,6 +39,7 @@ double gpu_upper_csr(const char* where, int n, const uint64_t* rowptr, const int
     }
     sptrsv_params_t prm{};
     prm.variant = variant;
+    prm.exact = exact;
     sptrsv_plan_timing(plan, 1);
     rc = sptrsv_solve_host(plan, &prm, b, x, rhs, col_major);
     double ms = 0.0;
@@ -130,12 +132,14 @@ double base_gauss_lower(int n, double* val, uint64_t* row_index, int* col, doubl
     return t;
 }
 
+// The serial solver's result is the contract here: AUTO restricted to the kernels that sum in
+// its order and divide like it (block kernel on grid factors, level-ordered pull otherwise).
 double base_gauss_upper(int n, double* val, uint64_t* row_index, int* col, double* x, double* b, int rhs) {
-    return gpu_upper_csr("base_gauss_upper", n, row_index, col, val, x, b, rhs, 1, SPTRSV_VAR_ROW_PULL);
+    return gpu_upper_csr("base_gauss_upper", n, row_index, col, val, x, b, rhs, 1, SPTRSV_VAR_AUTO, 1);
 }
 
 double base_gauss_upper_new(int n, double* val, uint64_t* row_index, int* col, double* x, double* b, int rhs) {
-    return gpu_upper_csr("base_gauss_upper_new", n, row_index, col, val, x, b, rhs, 1, SPTRSV_VAR_ROW_PULL);
+    return gpu_upper_csr("base_gauss_upper_new", n, row_index, col, val, x, b, rhs, 1, SPTRSV_VAR_AUTO, 1);
 }
 
 double supernodal_upper(size_t sn, int* supernodes, double* x, double* val, int* row, uint64_t* col_index, int n,
@@ -264,8 +268,10 @@ static std::string extension_of(const char* filename) {
     return dot == std::string::npos ? std::string() : f.substr(dot + 1);
 }
 
-double read_csr(const char* filename, int* n, unsigned long long* nz, uint64_t** row, int** col, double** val) {
-    const double t0 = now_s();
+// The reference's reader (src/utils.cpp:9-82) with a status: SPTRSV_ERR_IO for a file that cannot
+// be opened, an unknown extension or a short read, SPTRSV_ERR_BAD_INPUT for malformed contents.
+static int read_csr_status(const char* filename, int* n, unsigned long long* nz, uint64_t** row, int** col,
+                           double** val) {
     const std::string ext = extension_of(filename);
     *row = nullptr;
     *col = nullptr;
@@ -274,7 +280,7 @@ double read_csr(const char* filename, int* n, unsigned long long* nz, uint64_t**
         FILE* f = fopen(filename, "rb");
         if (!f) {
             fprintf(stderr, "read_csr: cannot open %s\n", filename);
-            return -1;
+            return SPTRSV_ERR_IO;
         }
         bool ok = fread(n, sizeof(int), 1, f) == 1 && fread(nz, sizeof(unsigned long long), 1, f) == 1 && *n > 0;
         if (ok) {
@@ -288,26 +294,27 @@ double read_csr(const char* filename, int* n, unsigned long long* nz, uint64_t**
         fclose(f);
         if (!ok) {
             fprintf(stderr, "read_csr: short read in %s\n", filename);
-            return -1;
+            return SPTRSV_ERR_IO;
         }
     } else if (ext == "mtx") {
         // coordinate entries sorted by column: a lower factor L read as the CSR of L^T
         std::ifstream file(filename);
         if (!file) {
             fprintf(stderr, "read_csr: cannot open %s\n", filename);
-            return -1;
+            return SPTRSV_ERR_IO;
         }
         while (file.peek() == '%') file.ignore(1 << 20, '\n');
         int n2 = 0;
         file >> *n >> n2 >> *nz;
-        if (!file || *n <= 0) return -1;
+        if (!file || *n <= 0) return SPTRSV_ERR_BAD_INPUT;
         *col = (int*)malloc(*nz * sizeof(int));
         *row = (uint64_t*)calloc((size_t)*n + 1, sizeof(uint64_t));
         *val = (double*)malloc(*nz * sizeof(double));
         for (unsigned long long k = 0; k < *nz; ++k) {
             int r = 0, c = 0;
             file >> r >> c >> (*val)[k];
-            if (!file || c < 1 || c > *n) return -1;
+            if (!file) return SPTRSV_ERR_IO;
+            if (c < 1 || c > *n) return SPTRSV_ERR_BAD_INPUT;
             (*col)[k] = r - 1;
             (*row)[c]++;  // entries of CSR row c-1
         }
@@ -316,22 +323,27 @@ double read_csr(const char* filename, int* n, unsigned long long* nz, uint64_t**
         std::ifstream file(filename);
         if (!file) {
             fprintf(stderr, "read_csr: cannot open %s\n", filename);
-            return -1;
+            return SPTRSV_ERR_IO;
         }
         file >> *n >> *nz;
-        if (!file || *n <= 0) return -1;
+        if (!file || *n <= 0) return SPTRSV_ERR_BAD_INPUT;
         *col = (int*)malloc(*nz * sizeof(int));
         *row = (uint64_t*)malloc(((size_t)*n + 1) * sizeof(uint64_t));
         *val = (double*)malloc(*nz * sizeof(double));
         for (unsigned long long k = 0; k < *nz; ++k) file >> (*val)[k];
         for (unsigned long long k = 0; k < *nz; ++k) file >> (*col)[k];
         for (int i = 0; i <= *n; ++i) file >> (*row)[i];
-        if (!file) return -1;
+        if (!file) return SPTRSV_ERR_IO;
     } else {
         fprintf(stderr, "\nUnknown extension of matrix file %s .Exit\n", ext.c_str());
-        return -1;
+        return SPTRSV_ERR_IO;
     }
-    return now_s() - t0;
+    return SPTRSV_OK;
+}
+
+double read_csr(const char* filename, int* n, unsigned long long* nz, uint64_t** row, int** col, double** val) {
+    const double t0 = now_s();
+    return read_csr_status(filename, n, nz, row, col, val) == SPTRSV_OK ? now_s() - t0 : -1;
 }
 
 void read_snodes(const char* filename, int* sn, int** snodes) {
@@ -461,8 +473,9 @@ double sptrsv_syncfree3_opencl(int* csrColIdx, int* csrRowPtr, const VALUE_TYPE*
     }
     std::vector<uint64_t> rp(csrRowPtr, csrRowPtr + n + 1);
     (void)nnz;
+    // the write-first kernel computes each row in the serial order: the exact AUTO choice
     return gpu_upper_csr("sptrsv_syncfree3_opencl", n, rp.data(), csrColIdx, csrVal, x, b, 1, 0,
-                         SPTRSV_VAR_ROW_PULL);
+                         SPTRSV_VAR_AUTO, 1);
 }
 
 // ---- C-ABI access to the loaders (for FFI callers; arrays are malloc'ed, free with
@@ -475,13 +488,14 @@ extern "C" int sptrsv_read_matrix_file(const char* path, int* n, int64_t* nnz, i
     *rowptr = nullptr;
     *col = nullptr;
     *val = nullptr;
-    if (read_csr(path, n, &nz, &row, col, val) < 0) {
+    const int rc = read_csr_status(path, n, &nz, &row, col, val);
+    if (rc != SPTRSV_OK) {
         free(row);
         free(*col);
         free(*val);
         *col = nullptr;
         *val = nullptr;
-        return SPTRSV_ERR_BAD_INPUT;
+        return rc;
     }
     static_assert(sizeof(uint64_t) == sizeof(int64_t), "row pointer width");
     *nnz = (int64_t)nz;
@@ -491,6 +505,15 @@ extern "C" int sptrsv_read_matrix_file(const char* path, int* n, int64_t* nnz, i
 
 extern "C" int sptrsv_read_snodes_file(const char* path, int* sn, int** snodes) {
     if (!path || !sn || !snodes) return SPTRSV_ERR_BAD_INPUT;
+    const std::string ext = extension_of(path);
+    FILE* f = fopen(path, "rb");
+    if (!f || (ext != "snodes" && ext != "txt")) {
+        if (f) fclose(f);
+        *snodes = nullptr;
+        *sn = 0;
+        return SPTRSV_ERR_IO;
+    }
+    fclose(f);
     read_snodes(path, sn, snodes);
     return *snodes ? SPTRSV_OK : SPTRSV_ERR_BAD_INPUT;
 }

@@ -145,10 +145,20 @@ int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double*
 void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip
 int supernodal_residual32(sptrsv_plan_s* p, const double* d_b, const double* d_x, float* d_r32, int m);
 int residual32(sptrsv_plan_s* p, const double* d_b, const double* d_x, float* d_r32, int m);
+struct RefineResult {
+    int iters;        // corrections applied
+    int outcome;      // SPTRSV_REFINE_*
+    double estimate;  // error estimate of the last iterate
+};
 int refine_loop(sptrsv_plan_s* p, const sptrsv_refine_t& ro, const double* d_b, double* d_x, int m,
                 const std::function<int(const float*, float*)>& solve_low,
-                const std::function<int(const double*, const double*, float*)>& residual, int* iters,
-                double* ratio);
+                const std::function<int(const double*, const double*, float*)>& residual, RefineResult* res);
+// AUTO's choice (exact: only kernels bitwise equal to the serial solver)
+int default_variant(sptrsv_plan_s* p, int m, int precision, bool exact);  // solve.hip
+// FP32 solve of device b32 -> x32 with the variant in params (not AUTO), m right-hand sides
+int solve_fp32_device(sptrsv_plan_s* p, const sptrsv_params_t* params, const float* d_b32, float* d_x32,
+                      int m);  // solve.hip
+int ensure_stage(sptrsv_plan_s* p, int64_t nm);  // solve.hip: FP32 staging buffers b32 / x32
 int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
                          uint64_t limit);  // supernodal.hip
 int tile_solve(sptrsv_plan_s* p, int precision, int rows, const double* d_b, double* d_x, int m,

@@ -246,10 +246,12 @@ __global__ __launch_bounds__(64) void k_push(PushArgs<T> a) {
     const int cs = a.colptr[j], ce = a.colptr[j + 1] - 1;
     const T diag = a.val[ce];
     if (lane == 0) {
+        // relaxed polls (an acquire load invalidates the L2 on every iteration); the single
+        // acquire fence below orders the column's reads after the last decrement it saw
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
         int idle = 0;
-        while (__hip_atomic_load(&a.indeg[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 1) {
+        while (__hip_atomic_load(&a.indeg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1) {
             __builtin_amdgcn_s_sleep(1);
             if ((++idle & 63) == 0 && wd.expired()) break;
         }
@@ -342,55 +344,6 @@ __global__ __launch_bounds__(256) void k_residual(int n, int m, const int* __res
     }
 }
 
-
-// ---- FP64 iterative refinement around reduced-precision solves ------------------------------
-
-// r32 = (float)(b - U x): FP64 products summed in ascending order, rounded once to FP32
-__global__ __launch_bounds__(256) void k_residual32(int n, int m, const int* __restrict__ rowptr,
-                                                    const int* __restrict__ col, const double* __restrict__ val,
-                                                    const double* __restrict__ x, const double* __restrict__ b,
-                                                    float* __restrict__ r32) {
-    const int64_t total = (int64_t)n * m;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
-        const int row = (int)(q / m), k = (int)(q % m);
-        double acc = 0.0;
-        for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
-        r32[q] = (float)(b[q] - acc);
-    }
-}
-
-// x += d (FP64) and per-column max|d|, max|x| (non-negative doubles order like their bits).
-// grid.y covers columns in groups of 64; 4 row-waves per block reduce through LDS.
-__global__ __launch_bounds__(256) void k_update(int n, int m, const float* __restrict__ d, double* __restrict__ x,
-                                                unsigned long long* __restrict__ norms, int first) {
-    __shared__ double sd[4][64], sx[4][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = blockIdx.y * 64 + lane;
-    double md = 0.0, mx = 0.0;
-    if (k < m) {
-        for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
-            const size_t at = (size_t)row * m + k;
-            const double dv = (double)d[at];
-            const double xv = first ? dv : x[at] + dv;
-            x[at] = xv;
-            md = fmax(md, fabs(dv));
-            mx = fmax(mx, fabs(xv));
-        }
-    }
-    sd[wv][lane] = md;
-    sx[wv][lane] = mx;
-    __syncthreads();
-    if (wv == 0 && k < m) {
-        md = fmax(fmax(sd[0][lane], sd[1][lane]), fmax(sd[2][lane], sd[3][lane]));
-        mx = fmax(fmax(sx[0][lane], sx[1][lane]), fmax(sx[2][lane], sx[3][lane]));
-        atomicMax(&norms[k], __builtin_bit_cast(unsigned long long, md));
-        atomicMax(&norms[m + k], __builtin_bit_cast(unsigned long long, mx));
-    }
-}
-
-__global__ void k_zero_u64(unsigned long long* p, int count) {
-    for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0ull;
-}
 
 inline int grid_for(int64_t count, int block = 256, int cap = 65536) {
     int64_t g = (count + block - 1) / block;
@@ -681,78 +634,42 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // LDS instead of paying a device-wide round trip per level; one to three right-hand sides in
 // FP64 over a natural-order grid factor to the block kernel (bricks solved in LDS; with four,
 // its per-step work outgrows the level-ordered pull kernel's); everything else to LEVEL_PULL.
-int default_variant(sptrsv_plan_s* P, int m, int precision) {
-    if (m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
+// With `exact`, only kernels whose FP64 x is bitwise equal to the serial solver qualify.
+int default_variant(sptrsv_plan_s* P, int m, int precision, bool exact) {
+    if (!exact && m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
     if (m <= 3 && precision == SPTRSV_FP64 && P->grid && block_supports(P, m)) return SPTRSV_VAR_BLOCK;
     return SPTRSV_VAR_LEVEL_PULL;
 }
 
-// r = b - U x over the plan's CSR in FP64, stored FP32 (any factor; one thread per (row, rhs)).
-int residual32(sptrsv_plan_s* P, const double* d_b, const double* d_x, float* d_r32, int m) {
-    const int64_t nm = (int64_t)P->n * m;
-    k_residual32<<<grid_for(nm), 256, 0, P->stream>>>(P->n, m, P->rowptr, P->col, P->val, d_x, d_b, d_r32);
-    SPT_HIP(hipGetLastError());
+int ensure_stage(sptrsv_plan_s* P, int64_t nm) {
+    if ((int64_t)P->stage_elems >= nm) return SPTRSV_OK;
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    if (P->b32) SPT_HIP(hipFree(P->b32));
+    if (P->x32) SPT_HIP(hipFree(P->x32));
+    P->b32 = P->x32 = nullptr;
+    P->stage_elems = 0;
+    SPT_HIP(hipMalloc((void**)&P->b32, nm * sizeof(float)));
+    SPT_HIP(hipMalloc((void**)&P->x32, nm * sizeof(float)));
+    P->stage_elems = (size_t)nm;
     return SPTRSV_OK;
 }
 
-// FP64 iterative refinement around a reduced-precision solve (see sptrsv_solve_refined).
-// Stopping rule: correction k has relative size r_k ~ the error of x before it; with the
-// observed contraction c = r_k / r_{k-1} the error left after it is about r_k * c.  Stop when
-// that estimate is <= tol, or when the corrections stop shrinking (the FP64 residual floor).
-int refine_loop(sptrsv_plan_s* P, const sptrsv_refine_t& ro, const double* d_b, double* d_x, int m,
-                const std::function<int(const float*, float*)>& solve_low,
-                const std::function<int(const double*, const double*, float*)>& residual, int* iters,
-                double* ratio) {
-    const int max_iters = ro.max_iters > 0 ? ro.max_iters : 30;
-    const double tol = ro.tol > 0 ? ro.tol : 1e-13;
-    const int n = P->n;
-    const int64_t nm = (int64_t)n * m;
-    hipStream_t s = P->stream;
-    if ((int64_t)P->ref_elems < nm || (int)P->ref_cols < m) {
-        if (P->ref_r32) SPT_HIP(hipFree(P->ref_r32));
-        if (P->ref_d32) SPT_HIP(hipFree(P->ref_d32));
-        if (P->ref_norms) SPT_HIP(hipFree(P->ref_norms));
-        P->ref_r32 = P->ref_d32 = nullptr;
-        P->ref_norms = nullptr;
-        SPT_HIP(hipMalloc((void**)&P->ref_r32, nm * sizeof(float)));
-        SPT_HIP(hipMalloc((void**)&P->ref_d32, nm * sizeof(float)));
-        SPT_HIP(hipMalloc((void**)&P->ref_norms, 2 * (size_t)m * sizeof(unsigned long long)));
-        P->ref_elems = (size_t)nm;
-        P->ref_cols = (size_t)m;
+// FP32 values, device FP32 b and x (the inner solve of the refinement, and sptrsv_solve at FP32).
+int solve_fp32_device(sptrsv_plan_s* P, const sptrsv_params_t* prm, const float* d_b32, float* d_x32, int m) {
+    const int variant = prm->variant;
+    const uint64_t limit = prm->watchdog_us > 0
+                               ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm->watchdog_us / 1000)
+                               : P->ticks_per_ms * 20000ull;
+    SPT_CHECK(ensure_fp32(P));
+    if (variant == SPTRSV_VAR_CHAIN) {
+        if (m != 1) return fail(SPTRSV_ERR_UNSUPPORTED, "chain variant solves one right-hand side");
+        return chain_solve32(P, d_b32, d_x32);
     }
-    const dim3 ugrid((unsigned)std::min<int64_t>(std::max<int64_t>((n + 3) / 4, 1), 4096), (unsigned)((m + 63) / 64));
-    std::vector<unsigned long long> h((size_t)2 * m);
-    k_convert<double, float><<<grid_for(nm), 256, 0, s>>>(d_b, P->ref_r32, nm);
-    SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
-    k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
-    k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 1);
-    SPT_HIP(hipGetLastError());
-    SPT_CHECK(sptrsv_plan_sync(P));
-    double last = 1.0, prev = 1.0, estimate = 1.0;
-    int done = 0;
-    for (int it = 1; it <= max_iters; ++it) {
-        SPT_CHECK(residual(d_b, d_x, P->ref_r32));
-        SPT_CHECK(solve_low(P->ref_r32, P->ref_d32));
-        k_zero_u64<<<1, 256, 0, s>>>(P->ref_norms, 2 * m);
-        k_update<<<ugrid, 256, 0, s>>>(n, m, P->ref_d32, d_x, P->ref_norms, 0);
-        SPT_HIP(hipGetLastError());
-        SPT_CHECK(sptrsv_plan_sync(P));
-        SPT_HIP(hipMemcpy(h.data(), P->ref_norms, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        last = 0.0;
-        for (int k = 0; k < m; ++k) {
-            const double dn = __builtin_bit_cast(double, h[(size_t)k]);
-            const double xn = __builtin_bit_cast(double, h[(size_t)m + k]);
-            last = std::max(last, xn > 0 ? dn / xn : dn);
-        }
-        done = it;
-        estimate = last * std::min(1.0, last / prev);
-        if (!(estimate > tol)) break;
-        if (it >= 2 && last > 0.5 * prev) break;
-        prev = last;
-    }
-    if (iters) *iters = done;
-    if (ratio) *ratio = estimate;
-    return SPTRSV_OK;
+    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK || variant == SPTRSV_VAR_SUPERNODAL)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "this variant has no FP32 form through sptrsv_solve");
+    const int unroll = prm->unroll > 0 ? prm->unroll : 4;
+    return solve_typed<float>(P, variant, prm->group, unroll, prm->occupancy, prm->block, d_b32, d_x32, m, P->val32,
+                              P->cval32, P->lval32, limit);
 }
 
 }  // namespace sptrsv
@@ -764,7 +681,7 @@ extern "C" {
 int sptrsv_plan_auto_variant(sptrsv_plan_t P, int m, int precision, int* variant) {
     if (!P || !variant) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
     if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
-    *variant = default_variant(P, m, precision);
+    *variant = default_variant(P, m, precision, false);
     return SPTRSV_OK;
 }
 
@@ -773,7 +690,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
     sptrsv_params_t prm{};
     if (params) prm = *params;
-    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m, prm.precision) : prm.variant;
+    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m, prm.precision, prm.exact != 0) : prm.variant;
     if (variant < 1 || variant >= SPTRSV_VAR_COUNT)
         return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
     const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
@@ -782,23 +699,17 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, prm.block, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_BLOCK) return block_solve(P, prm.precision, prm.occupancy, d_b, d_x, m, limit);
-    if (variant == SPTRSV_VAR_CHAIN) return chain_solve(P, prm.precision, d_b, d_x, m);
+    if (variant == SPTRSV_VAR_CHAIN && prm.precision == SPTRSV_FP64) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, prm.occupancy, prm.block, d_b, d_x, m, P->val, P->cval, P->lval, limit);
     if (prm.precision == SPTRSV_FP32) {
         SPT_CHECK(ensure_fp32(P));
         const int64_t nm = (int64_t)P->n * m;
-        if ((int64_t)P->stage_elems < nm) {
-            if (P->b32) SPT_HIP(hipFree(P->b32));
-            if (P->x32) SPT_HIP(hipFree(P->x32));
-            P->b32 = P->x32 = nullptr;
-            SPT_HIP(hipMalloc((void**)&P->b32, nm * sizeof(float)));
-            SPT_HIP(hipMalloc((void**)&P->x32, nm * sizeof(float)));
-            P->stage_elems = (size_t)nm;
-        }
+        SPT_CHECK(ensure_stage(P, nm));
+        sptrsv_params_t low = prm;
+        low.variant = variant;
         k_convert<double, float><<<grid_for(nm), 256, 0, P->stream>>>(d_b, P->b32, nm);
-        SPT_CHECK(solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, P->b32, P->x32, m, P->val32, P->cval32,
-                                     P->lval32, limit));
+        SPT_CHECK(solve_fp32_device(P, &low, P->b32, P->x32, m));
         k_convert<float, double><<<grid_for(nm), 256, 0, P->stream>>>(P->x32, d_x, nm);
         SPT_HIP(hipGetLastError());
         return SPTRSV_OK;
@@ -846,40 +757,6 @@ int sptrsv_solve_host(sptrsv_plan_t P, const sptrsv_params_t* params, const doub
     return rc;
 }
 
-
-int sptrsv_solve_refined(sptrsv_plan_t P, const sptrsv_params_t* params, const sptrsv_refine_t* opts,
-                         const double* d_b, double* d_x, int m, int* iters, double* ratio) {
-    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
-    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
-    sptrsv_params_t prm{};
-    if (params) prm = *params;
-    sptrsv_refine_t ro{};
-    if (opts) ro = *opts;
-    const int low = ro.precision == 0 ? SPTRSV_FP32 : ro.precision;
-    int variant = prm.variant == SPTRSV_VAR_AUTO ? default_variant(P, m, low) : prm.variant;
-    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK)
-        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the tile and block variants have no reduced-precision form");
-    if (variant == SPTRSV_VAR_CHAIN && m != 1)
-        return fail(SPTRSV_ERR_UNSUPPORTED, "refinement: the chain variant solves one right-hand side");
-    if (variant == SPTRSV_VAR_SUPERNODAL ? low != SPTRSV_BF16 : low != SPTRSV_FP32)
-        return fail(SPTRSV_ERR_UNSUPPORTED,
-                    "refinement: BF16 inner solves need the supernodal variant, the others run FP32");
-    const uint64_t limit = prm.watchdog_us > 0 ? std::max<uint64_t>(1, P->ticks_per_ms * (uint64_t)prm.watchdog_us / 1000)
-                                               : P->ticks_per_ms * 20000ull;
-    const int unroll = prm.unroll > 0 ? prm.unroll : 4;
-    if (variant != SPTRSV_VAR_SUPERNODAL) SPT_CHECK(ensure_fp32(P));
-    auto solve_low = [&](const float* in, float* out) -> int {
-        if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve_low(P, low, in, out, m, limit);
-        if (variant == SPTRSV_VAR_CHAIN) return chain_solve32(P, in, out);
-        return solve_typed<float>(P, variant, prm.group, unroll, prm.occupancy, prm.block, in, out, m, P->val32,
-                                  P->cval32, P->lval32, limit);
-    };
-    auto residual = [&](const double* b, const double* x, float* r) -> int {
-        if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_residual32(P, b, x, r, m);
-        return residual32(P, b, x, r, m);
-    };
-    return refine_loop(P, ro, d_b, d_x, m, solve_low, residual, iters, ratio);
-}
 
 int sptrsv_solve_transposed(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d_b, double* d_x,
                             int m) {

@@ -189,8 +189,27 @@ def test_c_abi_loaders(tmp_path):
     n, rp, col, val = capi.read_matrix(str(tmp_path / "s.matrix"))
     assert n == U.n and np.array_equal(rp, U.rowptr) and np.array_equal(col, U.col) and np.array_equal(val, U.val)
     assert np.array_equal(capi.read_snodes(str(tmp_path / "s.snodes")), U.snodes)
-    with pytest.raises(capi.SptrsvError):
-        capi.read_matrix(str(tmp_path / "missing.matrix"))
+    # file-level failures are SPTRSV_ERR_IO, malformed contents SPTRSV_ERR_BAD_INPUT
+    for bad in ("missing.matrix", "missing.txt", "missing.mtx"):
+        with pytest.raises(capi.SptrsvError) as e:
+            capi.read_matrix(str(tmp_path / bad))
+        assert e.value.code == capi.ERR_IO, bad
+    (tmp_path / "m.unknown").write_text("1 1\n1.0\n0\n0 1\n")
+    with pytest.raises(capi.SptrsvError) as e:
+        capi.read_matrix(str(tmp_path / "m.unknown"))
+    assert e.value.code == capi.ERR_IO
+    with open(tmp_path / "short.matrix", "wb") as f:
+        f.write(struct.pack("<iQ", 4, 10))  # header only: a short read
+    with pytest.raises(capi.SptrsvError) as e:
+        capi.read_matrix(str(tmp_path / "short.matrix"))
+    assert e.value.code == capi.ERR_IO
+    (tmp_path / "neg.txt").write_text("-3 1\n")
+    with pytest.raises(capi.SptrsvError) as e:
+        capi.read_matrix(str(tmp_path / "neg.txt"))
+    assert e.value.code == capi.ERR_BAD_INPUT
+    with pytest.raises(capi.SptrsvError) as e:
+        capi.read_snodes(str(tmp_path / "missing.snodes"))
+    assert e.value.code == capi.ERR_IO
 
 
 def test_tables_layout(tmp_path, monkeypatch):

@@ -579,15 +579,17 @@ def _refined(P, b, m, variant, precision, tol=0.0, max_iters=0):
     return d_x.download(), it, ratio
 
 
-@pytest.mark.parametrize("variant", [capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET, capi.VAR_CSC_PUSH,
-                                     capi.VAR_CHAIN])
-@pytest.mark.parametrize("m", [1, 8])
+REFINE_CASES = [(v, m) for m in (1, 8)
+                for v in (capi.VAR_LEVEL_PULL, capi.VAR_ROW_PULL, capi.VAR_LEVELSET, capi.VAR_CSC_PUSH,
+                          capi.VAR_CHAIN, capi.VAR_AUTO)
+                if not (v == capi.VAR_CHAIN and m > 1)]  # the chain kernel solves one right-hand side
+
+
+@pytest.mark.parametrize("variant,m", REFINE_CASES)
 def test_fp32_refined_reaches_fp64(plans, variant, m):
-    if variant == capi.VAR_CHAIN and m > 1:
-        pytest.skip("the chain kernel solves one right-hand side")
     for name in ("zipf_3000", "ic0_12", "chol2d_15", "snode_10"):
         U, P = MATS[name], plans[name]
-        banner(f"refine fp32 {name} m={m} {capi.VARIANT_NAMES[variant]}")
+        banner(f"refine fp32 {name} m={m} {capi.VARIANT_NAMES.get(variant, 'auto')}")
         xt, b = gen.rhs_for(U, m, seed=10, layout="row")
         x, it, est = _refined(P, b, m, variant, capi.FP32)
         assert est <= 1e-13 and 1 <= it <= 4, (name, it, est)
@@ -620,9 +622,16 @@ def test_refinement_reports_non_convergence():
         xt, b = gen.rhs_for(U, 1, seed=1)
         x, it, est = _refined(P, b, 1, capi.VAR_LEVEL_PULL, capi.FP32, max_iters=3)
         ref = oracle_solve(U, b)
-        assert 1 <= it <= 3
+        assert 0 <= it <= 3  # 0: the first FP32 iterate was already not finite
         if O.rel_error(ref, x) > FP64_TOL:
-            assert est > 1e-13, (it, est)
+            assert not (est <= 1e-13), (it, est)
+        d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+        info = P.solve_refined_ex(d_b, d_x, 1, capi.params(variant=capi.VAR_LEVEL_PULL), max_iters=3)
+        assert info["outcome"] != "converged", info
+        info = P.solve_refined_ex(d_b, d_x, 1, capi.params(variant=capi.VAR_LEVEL_PULL), max_iters=3, fallback=True)
+        assert info["fell_back"], info
+        # x is the FP64 AUTO solve (the chain kernel here, deterministic: one wave)
+        assert np.array_equal(d_x.download(), P.solve_host(b, capi.params()))
 
 
 def test_refinement_rejects_unsupported(plans):
@@ -632,3 +641,63 @@ def test_refinement_rejects_unsupported(plans):
         with pytest.raises(capi.SptrsvError) as e:
             _refined(P, b, 1, variant, prec)
         assert e.value.code == capi.ERR_UNSUPPORTED
+
+
+def test_refinement_overflow_is_divergence_and_falls_back():
+    """A factor whose solution overflows FP32 (x grows 1000x per row along a chain) but not FP64:
+    the FP32 solve yields inf/NaN, which the norms must keep (a max that drops NaN would report
+    convergence).  The refinement reports divergence; with the fallback x is the FP64 solve."""
+    n = 60
+    rows = np.concatenate([np.arange(n), np.arange(n - 1)])
+    cols = np.concatenate([np.arange(n), np.arange(1, n)])
+    vals = np.concatenate([np.full(n, 1e-3), np.full(n - 1, 1.0)])
+    o = np.lexsort((cols, rows))
+    U = gen._from_rows(n, rows[o], cols[o], vals[o], "overflow_chain")
+    b = np.ones(n)
+    ref = oracle_solve(U, b)
+    assert np.all(np.isfinite(ref)) and np.abs(ref).max() > 1e100
+    with capi.Plan.from_upper(U) as P:
+        for variant in (capi.VAR_LEVEL_PULL, capi.VAR_CHAIN):
+            d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+            info = P.solve_refined_ex(d_b, d_x, 1, capi.params(variant=variant))
+            assert info["outcome"] == "diverged" and not info["fell_back"], info
+            assert not np.isfinite(info["estimate"]) or info["estimate"] > 1e-13, info
+            it, est = P.solve_refined(d_b, d_x, 1, capi.params(variant=variant))
+            assert not (est <= 1e-13), (it, est)
+            info = P.solve_refined_ex(d_b, d_x, 1, capi.params(variant=variant), fallback=True)
+            assert info["fell_back"], info
+            assert O.rel_error(ref, d_x.download()) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["chol2d_15", "zipf_3000", "chain_4000", "csr_5"])
+def test_auto_fp32_chain_like(plans, name):
+    """AUTO at FP32 picks the chain kernel on chain-like factors and runs it in FP32."""
+    U, P = MATS[name], plans[name]
+    assert P.n_levels * 4 > U.n and P.auto_variant(1, capi.FP32) == capi.VAR_CHAIN
+    xt, b = gen.rhs_for(U, 1, seed=3)
+    x = P.solve_host(b, capi.params(precision=capi.FP32))
+    assert O.rel_error(oracle_solve(U, b), x) <= FP32_TOL
+
+
+@pytest.mark.parametrize("m", [1, 2, 5])
+@pytest.mark.parametrize("name", list(MATS))
+def test_exact_auto_is_bitwise(plans, name, m):
+    """params.exact: AUTO restricted to kernels that reproduce the serial solver bit for bit."""
+    U, P = MATS[name], plans[name]
+    xt, b = gen.rhs_for(U, m, seed=31, layout="row")
+    x = P.solve_host(b, capi.params(exact=True))
+    assert np.array_equal(x, oracle_solve(U, b))
+
+
+@pytest.mark.parametrize("name", ["csr_5", "zipf_3000", "ic0_12", "chol2d_15", "dense_70", "chain_4000"])
+def test_condest_against_dense(plans, name):
+    """kappa_1 estimate against the dense 1-norm condition number (numpy): a lower bound, and on
+    these factors within a factor 3 (Higham's estimator is usually exact)."""
+    U, P = MATS[name], plans[name]
+    if U.n > 4000:
+        pytest.skip("dense reference too large")
+    D = U.to_scipy().toarray()
+    true = np.linalg.norm(D, 1) * np.linalg.norm(np.linalg.inv(D), 1)
+    kappa, ninv = P.condest()
+    assert kappa <= true * (1 + 1e-8), (kappa, true)
+    assert kappa >= true / 3, (kappa, true)

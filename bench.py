@@ -181,34 +181,106 @@ def bench_mixed(capi, gen, names, out):
         progress(f"mixed {key}: {out[key]}")
 
 
-def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 4, 16)):
-    """Batched solve of `count` systems (n log-uniform [256, nmax]; banded / Zipf / supernodal)
-    in one persistent graph-replayed launch, for each m: the fallback configuration, exhaustive
-    autotuning, and the MLP selector (one network over all m, trained on the autotuner's labels
-    for a separate batch, seed 4); plus the FP32 + FP64-refinement batched solve."""
+KAPPA_TARGETS = (1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8)
+
+
+def bench_kappa(capi, gen, out, plan_ms):
+    """Mixed precision with FP64 refinement across the condition-number sweep: the Zipf factor
+    (n = 1e6) and the supernodal factor (4,096 supernodes) with the diagonal scaled so that
+    kappa_1 spans 1e2 ... 1e8 (gen.KAPPA_SCALES).  Per factor: kappa re-estimated on the GPU
+    (sptrsv_condest); per m: the plain FP64 solve, the refined solve (FP32 inner solves for Zipf,
+    BF16 rectangular products for the supernodal factor) with the FP64 fallback on, wall times
+    with data resident, and the refined x against the serial oracle.  plan_ms: family -> m list."""
+    from oracle import oracle as O
+    for fam, ms in plan_ms.items():
+        base = gen.random_zipf(1_000_000, seed=1) if fam == "zipf_1e6" else gen.random_supernodal(4096, seed=2)
+        sn = fam.startswith("supernodal")
+        for tgt in KAPPA_TARGETS:
+            key = f"{fam}/kappa{tgt:.0e}"
+            res = {"diag_scale": gen.KAPPA_SCALES[fam][tgt]}
+            try:
+                U = gen.scale_diagonal(base, gen.KAPPA_SCALES[fam][tgt])
+                t0 = time.perf_counter()
+                P = capi.Plan.from_upper(U)
+                if sn:
+                    P.build_supernodes(U.snodes)
+                res["plan_s"] = round(time.perf_counter() - t0, 3)
+                t0 = time.perf_counter()
+                res["kappa1_gpu"], _ = P.condest()
+                res["condest_s"] = round(time.perf_counter() - t0, 3)
+                for m in ms:
+                    xt, b = gen.rhs_for(U, m, seed=1, layout="row")
+                    ref = (O.base_gauss_upper(U.rowptr, U.col, U.val, b)[0] if m == 1 else
+                           np.ascontiguousarray(O.base_gauss_upper_columns(U.rowptr, U.col, U.val,
+                                                                           np.ascontiguousarray(b.T)).T))
+                    d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+                    prm64 = capi.params(variant=capi.VAR_SUPERNODAL if sn else capi.VAR_AUTO)
+                    P.solve(d_b, d_x, m, prm64)
+                    P.sync()
+                    t64 = []
+                    for _ in range(2):
+                        t0 = time.perf_counter()
+                        P.solve(d_b, d_x, m, prm64)
+                        P.sync()
+                        t64.append(time.perf_counter() - t0)
+                    e64 = O.rel_error(ref, d_x.download())
+                    prec = capi.BF16 if sn else capi.FP32
+                    prm_lo = capi.params(variant=capi.VAR_SUPERNODAL if sn else capi.VAR_AUTO)
+                    P.solve_refined_ex(d_b, d_x, m, prm_lo, precision=prec, fallback=True)  # warm-up
+                    tr, info = [], None
+                    for _ in range(2):
+                        t0 = time.perf_counter()
+                        info = P.solve_refined_ex(d_b, d_x, m, prm_lo, precision=prec, fallback=True)
+                        tr.append(time.perf_counter() - t0)
+                    err = O.rel_error(ref, d_x.download())
+                    res[f"m{m}"] = {"fp64_ms": round(min(t64) * 1e3, 3), "fp64_rel_err_vs_oracle": e64,
+                                    "refined_ms": round(min(tr) * 1e3, 3), "speedup": round(min(t64) / min(tr), 3),
+                                    "inner": "bf16+fp32" if sn else "fp32", "inner_variant":
+                                        capi.VARIANT_NAMES.get(info["variant"], "?"), "iters": info["iters"],
+                                    "outcome": info["outcome"], "fell_back": info["fell_back"],
+                                    "estimate": info["estimate"], "rel_err_vs_oracle": err}
+                    d_b.free()
+                    d_x.free()
+                P.close()
+            except Exception as exc:  # report, never hide
+                res["error"] = str(exc)
+            out[key] = res
+            progress(f"kappa {key}: {res}")
+
+
+SELECTOR_PATH = os.path.join(ROOT, "sptrsv_amd", "selector.json")
+
+
+def bench_batch(capi, gen, out, count=4096, nmax=65536, ms=(1, 4, 16)):
+    """Batched solve of `count` systems from the native generator (seed 3; n log-uniform
+    [256, nmax]; banded / Zipf / supernodal shaped like the 4,096-supernode factor) in one
+    persistent graph-replayed launch, for each m: the plain fallback configuration (LS1),
+    exhaustive autotuning, the MLP selector (trained by `python -m sptrsv_amd.tune` on 20,000
+    systems of seed 4, loaded from sptrsv_amd/selector.json), the selector with its longest
+    systems re-decided, and the FP32 + FP64-refinement solve with the selector's choices.
+    b = U x_true is formed on the device (the batch residual with b = 0 gives -U x_true)."""
     from sptrsv_amd import tune
-    Ut, spt, _ = gen.batch_systems(train_count, seed=4, nmin=256, nmax=nmax)
-    Bt = capi.Batch(Ut, spt)
-    feats_t, times_t, masks_t = [], [], []
-    for m in ms:
-        _, bt = gen.rhs_for(Ut, m, seed=2, layout="row")
-        dbt, dxt = capi.DeviceArray.from_host(bt), capi.DeviceArray(bt.shape)
-        T = tune.exhaustive(Bt, dbt, dxt, m=m, reps=2)
-        feats_t.append(Bt.features(m))
-        times_t.append(T)
-        dbt.free(); dxt.free()
-    Bt.close()
-    del Ut
-    sel = tune.Selector(seed=0)
-    loss = sel.fit_multi(feats_t, times_t, list(ms), epochs=800)
-    progress(f"batch: selector trained on {train_count} systems x m in {list(ms)}, final loss {loss:.4f}")
-    U, sp, kinds = gen.batch_systems(count, seed=3, nmin=256, nmax=nmax)
+    sel = tune.Selector.load(SELECTOR_PATH)
+    t0 = time.perf_counter()
+    U, sp, kinds = gen.batch_systems_native(count, seed=3, nmin=256, nmax=nmax)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
     B = capi.Batch(U, sp)
-    res = {"systems": count, "n_total": U.n, "nnz_total": U.nnz, "nmax": nmax, "train_systems": train_count,
-           "train_final_loss": loss}
+    t_create = time.perf_counter() - t0
+    res = {"systems": count, "n_total": U.n, "nnz_total": U.nnz, "nmax": nmax,
+           "kinds": {k: kinds.count(k) for k in gen.BATCH_KINDS}, "generate_s": round(t_gen, 1),
+           "batch_create_s": round(t_create, 1), "selector": sel.meta}
+    progress(f"batch: {count} systems, n={U.n}, nnz={U.nnz}, generated {t_gen:.1f} s, created {t_create:.1f} s")
+    rng = np.random.default_rng(1)
     for m in ms:
-        _, b = gen.rhs_for(U, m, seed=1, layout="row")
-        d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+        shape = (U.n, m) if m > 1 else (U.n,)
+        xt = rng.uniform(-1.0, 1.0, shape)
+        d_xt = capi.DeviceArray.from_host(xt)
+        d_0 = capi.DeviceArray.from_host(np.zeros(shape))
+        d_b, d_x = capi.DeviceArray(shape), capi.DeviceArray(shape)
+        B.residual(d_0, d_xt, d_b, m)  # d_b = -U x_true: the solution is -x_true
+        B.sync()
+        d_0.free()
         T = tune.exhaustive(B, d_b, d_x, m=m, reps=2)
         feats = B.features(m)
         mlp = sel.predict(feats, m=m)
@@ -216,12 +288,12 @@ def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 
                    "mlp": mlp, "mlp_guarded": tune.guard_longest(U, sp, feats, mlp, m)}
         rm = {}
         walls = {name: [] for name in choices}
-        for _ in range(3):  # the choices alternate, so a transient slows all of them alike
+        for _ in range(2):  # the choices alternate, so a transient slows all of them alike
             for name, ch in choices.items():
                 B.set_configs(ch, tune.lpt_order(T[np.arange(B.nsys), ch]))
                 B.solve(d_b, d_x, m, capi.FP64, True)
                 B.sync()
-                for _ in range(5 if name != "fallback_ls1" else 1):
+                for _ in range(3 if name != "fallback_ls1" else 1):
                     t0 = time.perf_counter()
                     B.solve(d_b, d_x, m, capi.FP64, True)
                     B.sync()
@@ -236,18 +308,27 @@ def bench_batch(capi, gen, out, count=2048, train_count=1024, nmax=4096, ms=(1, 
                         "regret": round(tune.regret(T, ch), 4)}
         rm["mlp_vs_exhaustive_throughput"] = round(rm["exhaustive"]["ms"] / rm["mlp"]["ms"], 4)
         rm["mlp_guarded_vs_exhaustive_throughput"] = round(rm["exhaustive"]["ms"] / rm["mlp_guarded"]["ms"], 4)
+        rm["mlp_vs_fallback_throughput"] = round(rm["fallback_ls1"]["ms"] / rm["mlp"]["ms"], 4)
         rm["mlp_accuracy"] = float((choices["mlp"] == choices["exhaustive"]).mean())
-        # FP32 solves + FP64 refinement with the selector's configurations
         ch = choices["mlp"]
         B.set_configs(ch, tune.lpt_order(T[np.arange(B.nsys), ch]))
-        B.solve_refined(d_b, d_x, m)
+        B.solve(d_b, d_x, m, capi.FP64, True)
+        B.sync()
+        x = d_x.download()
+        rm["fp64_forward_err_vs_xtrue"] = float(np.linalg.norm(x + xt) / np.linalg.norm(xt))
+        # FP32 solves + FP64 refinement with the selector's configurations, FP64 fallback on
+        B.solve_refined_ex(d_b, d_x, m, fallback=True)
         t0 = time.perf_counter()
-        it, est = B.solve_refined(d_b, d_x, m)
+        info = B.solve_refined_ex(d_b, d_x, m, fallback=True)
         t_ref = time.perf_counter() - t0
-        rm["fp32_refined"] = {"ms": round(t_ref * 1e3, 3), "iters": it, "error_estimate": est}
+        xr = d_x.download()
+        rm["fp32_refined"] = {"ms": round(t_ref * 1e3, 3), **info,
+                              "rel_err_vs_fp64": float(np.linalg.norm(xr - x) / np.linalg.norm(x))}
         res[f"m{m}"] = rm
         progress(f"batch m={m}: {rm}")
-        d_b.free(); d_x.free()
+        for d in (d_b, d_x, d_xt):
+            d.free()
+        del x, xr, xt
     out["batch"] = res
     sel.close()
     B.close()
@@ -268,6 +349,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run N-byte device copies (counter calibration)")
+    ap.add_argument("--suite", default="", help="kappa: only the full condition-number sweep (m in 1, 32, 128)")
+    ap.add_argument("--no-kappa", action="store_true")
     args = ap.parse_args()
     t_start = time.perf_counter()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -280,6 +363,12 @@ def main():
         return
     from oracle import oracle as O
     from sptrsv_amd import capi, gen
+
+    if args.suite == "kappa":
+        sweep = {}
+        bench_kappa(capi, gen, sweep, {"zipf_1e6": (1, 32, 128), "supernodal_4096": (1, 32, 128)})
+        print(json.dumps({"suite": "kappa", "kappa_sweep": sweep, "bench_wall_s": round(time.perf_counter() - t_start, 1)}))
+        return
 
     names = {v: k for k, v in capi.VARIANT_NAMES.items()}
     variant = names[args.variant]
@@ -406,9 +495,11 @@ def main():
     if cpu:
         progress(f"cpu baseline: {cpu['sample']}")
     del P
-    mixed, batch = {}, {}
+    mixed, batch, sweep = {}, {}, {}
     if not args.no_mixed:
         bench_mixed(capi, gen, names, mixed)
+    if not args.no_kappa:
+        bench_kappa(capi, gen, sweep, {"zipf_1e6": (1,), "supernodal_4096": (32,)})
     if not args.no_batch:
         try:
             bench_batch(capi, gen, batch)
@@ -448,6 +539,7 @@ def main():
         "cpu_baseline": cpu,
         "extras": extras,
         "mixed_refine": mixed,
+        "kappa_sweep": sweep,
         "batched": batch,
         "bench_wall_s": round(time.perf_counter() - t_start, 1),
     }

@@ -135,6 +135,15 @@ int sptrsv_plan_info(sptrsv_plan_t plan, int* n, int64_t* nnz, int* n_levels);
  * fundamental supernodes from the pattern.  Supernodes wider than 128 rows are split (the
  * split keeps the structure).  Diagonal blocks are inverted on the device. */
 int sptrsv_plan_build_supernodes(sptrsv_plan_t plan, int sn, const int* snodes);
+/* Relaxed supernodes (no reference counterpart): consecutive rows are merged while the block,
+ * padded with explicit zeros to a dense triangle plus one common column set, stays within
+ * max_width rows (0 -> 128) and (1 + max_fill) times the rows' own entries (0 -> 1.0).  Any
+ * factor qualifies; a banded factor becomes a chain of wide dense blocks solved on MFMA.  AUTO
+ * builds this layout by itself, once, for a chain-like factor with at least 16 entries per row
+ * on average when the blocks average at least 32 rows (and then solves FP64 through it). */
+int sptrsv_plan_build_supernodes_relaxed(sptrsv_plan_t plan, int max_width, double max_fill);
+/* Explicit zeros stored by the current supernodal layout (0 for exact supernodes). */
+int sptrsv_plan_supernodal_padding(sptrsv_plan_t plan, int64_t* explicit_zeros);
 /* Number of supernodes in use and (optionally, sn+1 entries) their boundaries. */
 int sptrsv_plan_get_supernodes(sptrsv_plan_t plan, int* sn, int* snodes);
 
@@ -160,8 +169,10 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
                         double* val /* nnz */);
 
 /* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
- * counterpart): CHAIN for m == 1 when levels hold < 4 rows on average (never with
- * params.exact), BLOCK for FP64 m <= 3 on a natural-order grid factor, LEVEL_PULL otherwise.  For 2 <= m <= 3 on a grid
+ * counterpart).  For a chain-like factor (levels hold < 4 rows on average): SUPERNODAL in FP64
+ * when relaxed supernodes pay (see sptrsv_plan_build_supernodes_relaxed; built here on first
+ * use), else CHAIN for m == 1.  BLOCK for FP64 m <= 3 on a natural-order grid factor;
+ * LEVEL_PULL otherwise.  With params.exact only BLOCK and LEVEL_PULL are chosen.  For 2 <= m <= 3 on a grid
  * factor the choice checks the block layout's limits and so builds that layout (on the host,
  * once per plan; the block solve would build it on its first call). */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
@@ -254,7 +265,10 @@ int sptrsv_condest(sptrsv_plan_t plan, double* kappa, double* norm_inv);
 
 /* ---- batches of independent systems (no reference counterpart) -------------------------
  * The systems are given as one block-diagonal upper CSR: system s owns rows
- * sys_ptr[s] .. sys_ptr[s+1]-1 and only references its own columns.  Every system is solved by
+ * sys_ptr[s] .. sys_ptr[s+1]-1 and only references its own columns.  Row pointers are 64-bit
+ * end to end (a batch may hold more than 2^31 nonzeros); column indices are global rows (int).
+ * Creation validates every system, computes its levels and selector features on the host
+ * (systems in parallel) and uploads the matrix.  Every system is solved by
  * one persistent workgroup with its own configuration (SPTRSV_BCFG_*), systems popped from a
  * queue in a caller-given order; sptrsv_batch_solve can replay the launch as a captured
  * hipGraph.  Device x/b are row-major (n_total x m). */
@@ -288,8 +302,9 @@ int sptrsv_batch_solve_refined(sptrsv_batch_t batch, const sptrsv_refine_t* opts
                                int m, int* iters, double* estimate);
 int sptrsv_batch_solve_refined_ex(sptrsv_batch_t batch, const sptrsv_refine_t* opts, const double* d_b,
                                   double* d_x, int m, sptrsv_refine_info_t* info);
-/* The block-diagonal plan under the batch (for residuals, timing, refinement). */
-sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t batch);
+/* r = b - U x over every system, FP64, row-major (n_total x m), asynchronous on the batch's
+ * stream (sptrsv_batch_sync waits). */
+int sptrsv_batch_residual(sptrsv_batch_t batch, const double* d_b, const double* d_x, double* d_r, int m);
 /* Autotuner primitive: every system on `cfg` in one launch; device microseconds per system. */
 int sptrsv_batch_time_config(sptrsv_batch_t batch, int cfg, int precision, const double* d_b, double* d_x, int m,
                              double* sys_us);

@@ -20,7 +20,8 @@ LIB = os.path.join(LIBDIR, "libsptrsv_amd.so")
 ARCH = os.environ.get("SPTRSV_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["runtime.hip", "analysis.hip", "solve.hip", "supernodal.hip", "tile.hip", "chain.hip", "batch.hip", "mlp.hip", "block.hip", "refine.hip"]
-CXX_SOURCES = ["dropin.cpp"]
+CXX_SOURCES = ["dropin.cpp", "gen.cpp"]
+OPENMP_HIP_SOURCES = {"batch.hip"}  # host-side loops over systems run in parallel
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # packed FP32 ops are disabled for all device code (some of their forms have given wrong
@@ -29,7 +30,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Wno-unused-command-line-argument",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", os.path.join(ROOT, "include"), "-I", CSRC]
-CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-fopenmp", "-I", os.path.join(ROOT, "include")]
+CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-fopenmp", "-I", os.path.join(ROOT, "include")]
 
 
 def _run(cmd):
@@ -47,7 +48,8 @@ def build(verbose: bool = False) -> str:
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        extra = ["-fopenmp"] if src in OPENMP_HIP_SOURCES else []
+        jobs.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)

@@ -135,7 +135,7 @@ def lib():
         "sptrsv_batch_sync": (_i, [_vp]),
         "sptrsv_batch_solve_refined": (_i, [_vp, ctypes.POINTER(RefineOpts), _vp, _vp, _i, ctypes.POINTER(_i),
                                             ctypes.POINTER(ctypes.c_double)]),
-        "sptrsv_batch_plan": (_vp, [_vp]),
+        "sptrsv_batch_residual": (_i, [_vp, _vp, _vp, _vp, _i]),
         "sptrsv_batch_time_config": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp]),
         "sptrsv_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_uint, ctypes.POINTER(_vp)]),
         "sptrsv_mlp_destroy": (_i, [_vp]),
@@ -148,11 +148,15 @@ def lib():
         "sptrsv_solve_refined": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                       ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_plan_get_supernodes": (_i, [_PLAN, ctypes.POINTER(_i), _vp]),
+        "sptrsv_plan_build_supernodes_relaxed": (_i, [_PLAN, _i, ctypes.c_double]),
+        "sptrsv_plan_supernodal_padding": (_i, [_PLAN, ctypes.POINTER(_i64)]),
         "sptrsv_solve_refined_ex": (_i, [_PLAN, ctypes.POINTER(Params), ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                          ctypes.POINTER(RefineInfo)]),
         "sptrsv_batch_solve_refined_ex": (_i, [_vp, ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                                ctypes.POINTER(RefineInfo)]),
         "sptrsv_condest": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "sptrsv_gen_batch_sizes": (_i, [_i, _i, ctypes.c_uint64, _i, _i, _vp, _vp, _vp]),
+        "sptrsv_gen_batch_fill": (_i, [_i, _i, ctypes.c_uint64, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -397,6 +401,14 @@ class Plan:
             check(lib().sptrsv_plan_build_supernodes(self.handle, len(s) - 1, _ptr(s)), "build_supernodes")
         return self.supernodes()
 
+    def build_supernodes_relaxed(self, max_width=0, max_fill=0.0):
+        """Relaxed supernodes (explicit zeros allowed); returns (boundaries, explicit zeros)."""
+        check(lib().sptrsv_plan_build_supernodes_relaxed(self.handle, int(max_width), float(max_fill)),
+              "build_supernodes_relaxed")
+        z = _i64(0)
+        check(lib().sptrsv_plan_supernodal_padding(self.handle, ctypes.byref(z)), "supernodal_padding")
+        return self.supernodes(), z.value
+
     def supernodes(self):
         sn = _i(0)
         check(lib().sptrsv_plan_get_supernodes(self.handle, ctypes.byref(sn), None), "get_supernodes")
@@ -533,14 +545,16 @@ class Batch:
         self.handle = h
         self.nsys = len(sp) - 1
         self.n = U.n
-        self.plan = Plan(_PLAN(lib().sptrsv_batch_plan(h)))
-        self.plan._owned = False
+        self.nnz = len(c)
 
     def close(self):
         if self.handle:
-            self.plan.handle = None  # owned by the batch
             lib().sptrsv_batch_destroy(self.handle)
             self.handle = None
+
+    def residual(self, d_b, d_x, d_r, m=1):
+        """r = b - U x over every system (FP64, asynchronous; sync() waits)."""
+        check(lib().sptrsv_batch_residual(self.handle, d_b.ptr, d_x.ptr, d_r.ptr, int(m)), "batch_residual")
 
     def __del__(self):
         try:

@@ -1,8 +1,9 @@
 // batch.hip -- many independent upper-triangular systems in one persistent launch.
 //
-// The systems are stored as one block-diagonal factor (a plan over all rows: its analysis
-// already gives every row its level inside its own system).  Each system carries a
-// configuration:
+// The systems are stored as one block-diagonal CSR with 64-bit row pointers (a batch of
+// thousands of large systems passes 2^31 nonzeros), owned by the batch; the level analysis of
+// every system runs on the host when the batch is created (systems in parallel).  Each system
+// carries a configuration:
 //   LS_G1 / LS_G4 / LS_G16 / LS_G64   one workgroup walks the system's levels with a barrier
 //       between levels; G lanes per row (over the right-hand sides when m > 1, over the row's
 //       entries when m == 1 -- then the partial sums meet through cross-lane shuffles)
@@ -14,6 +15,7 @@
 // system's device time (s_memrealtime) -- the autotuner sweeps configurations with it, one
 // launch per configuration.  The whole solve is replayable as a captured hipGraph.
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <cmath>
@@ -24,9 +26,15 @@
 #include "plan.h"
 
 struct sptrsv_batch_s {
-    sptrsv_plan_s* plan = nullptr;
+    sptrsv_plan_s* ctl = nullptr;  // stream, status word, timing, refinement and FP32 staging
     int nsys = 0;
+    int n = 0;
+    int64_t nnz = 0;
     std::vector<int> h_sys_ptr;
+    int64_t* rowptr = nullptr;  // n + 1, block-diagonal CSR, diagonal first in every row
+    int* col = nullptr;
+    double* val = nullptr;
+    float* val32 = nullptr;     // FP32 copy, built on the first FP32 solve
     int* sys_ptr = nullptr;   // nsys + 1 row offsets
     int* lvl_off = nullptr;   // nsys + 1 offsets into lvl_ptr
     int* lvl_ptr = nullptr;   // per system: L_s + 1 absolute positions into rows[]
@@ -61,7 +69,7 @@ struct BatchArgs {
     const int* rows;
     const int* cfg;
     const int* order;
-    const int* rowptr;
+    const int64_t* rowptr;
     const int* col;
     const T* val;
     const T* b;
@@ -94,10 +102,10 @@ __device__ void chain_system(const BatchArgs<T>& a, int lo, int top, T (*tile)[6
         for (int s = 0; s < 64; ++s) tile[s][lane] = T(0);
         T acc = T(0), diag = T(1), bv = T(0);
         if (valid) {
-            const int st = a.rowptr[r], en = a.rowptr[r + 1];
+            const int64_t st = a.rowptr[r], en = a.rowptr[r + 1];
             diag = a.val[st];
             bv = a.b[r];
-            for (int p = st + 1; p < en; ++p) {
+            for (int64_t p = st + 1; p < en; ++p) {
                 const int c = a.col[p];
                 const T v = a.val[p];
                 if (c < hi) tile[hi - 1 - c][lane] = v;
@@ -142,9 +150,9 @@ __device__ void chain_system_m(const BatchArgs<T>& a, int lo, int top, T (*tile)
             for (int s = 0; s < 64; ++s) tile[s][lane] = T(0);
             T diag = T(1);
             if (valid) {
-                const int st = a.rowptr[r], en = a.rowptr[r + 1];
+                const int64_t st = a.rowptr[r], en = a.rowptr[r + 1];
                 diag = a.val[st];
-                for (int p = st + 1; p < en; ++p) {
+                for (int64_t p = st + 1; p < en; ++p) {
                     const int c = a.col[p];
                     if (c < hi) tile[hi - 1 - c][lane] = a.val[p];
                 }
@@ -157,9 +165,9 @@ __device__ void chain_system_m(const BatchArgs<T>& a, int lo, int top, T (*tile)
             T* rk = ring + k * R;
             T acc = T(0), bv = T(0);
             if (valid) {
-                const int st = a.rowptr[r], en = a.rowptr[r + 1];
+                const int64_t st = a.rowptr[r], en = a.rowptr[r + 1];
                 bv = a.b[(size_t)r * m + k];
-                for (int p = st + 1; p < en; ++p) {
+                for (int64_t p = st + 1; p < en; ++p) {
                     const int c = a.col[p];
                     if (c < hi) continue;
                     const T v = a.val[p];
@@ -219,12 +227,12 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs<T> a) {
                     const int idx = base + grp;
                     const bool on = idx < re;
                     const int row = on ? a.rows[idx] : 0;
-                    const int st = on ? a.rowptr[row] : 0, en = on ? a.rowptr[row + 1] : 0;
+                    const int64_t st = on ? a.rowptr[row] : 0, en = on ? a.rowptr[row + 1] : 0;
                     if (a.m == 1) {
                         // G lanes split the entries, cross-lane reduction (groups are whole
                         // aligned lane sets, so every lane of a group takes the same path)
                         T acc = T(0);
-                        for (int p = st + 1 + gl; p < en; p += G) acc = acc + a.val[p] * a.x[a.col[p]];
+                        for (int64_t p = st + 1 + gl; p < en; p += G) acc = acc + a.val[p] * a.x[a.col[p]];
                         for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
                         if (on && gl == 0) a.x[row] = (a.b[row] - acc) / a.val[st];
                     } else if (on) {
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs<T> a) {
                         const T d = a.val[st];
                         for (size_t k = gl; k < m; k += G) {
                             T acc = T(0);
-                            for (int p = st + 1; p < en; ++p) acc = acc + a.val[p] * a.x[(size_t)a.col[p] * m + k];
+                            for (int64_t p = st + 1; p < en; ++p) acc = acc + a.val[p] * a.x[(size_t)a.col[p] * m + k];
                             a.x[(size_t)row * m + k] = (a.b[(size_t)row * m + k] - acc) / d;
                         }
                     }
@@ -262,11 +270,43 @@ int upload(X** dst, const std::vector<X>& src) {
     return SPTRSV_OK;
 }
 
+// r = b - U x over the batch (FP64 products and sums); r32 rounds it once to FP32
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_residual(int n, int m, const int64_t* __restrict__ rowptr,
+                                                        const int* __restrict__ col, const double* __restrict__ val,
+                                                        const double* __restrict__ x, const double* __restrict__ b,
+                                                        R* __restrict__ r) {
+    const int64_t total = (int64_t)n * m;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int row = (int)(q / m), k = (int)(q % m);
+        double acc = 0.0;
+        for (int64_t e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
+        r[q] = (R)(b[q] - acc);
+    }
+}
+
+__global__ void k_val_to_float(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+inline int grid_for_b(int64_t count) {
+    return (int)std::min<int64_t>(std::max<int64_t>((count + 255) / 256, 1), 65536);
+}
+
+int ensure_batch_fp32(sptrsv_batch_s* B) {
+    if (B->val32) return SPTRSV_OK;
+    SPT_HIP(hipMalloc((void**)&B->val32, std::max<int64_t>(B->nnz, 1) * sizeof(float)));
+    k_val_to_float<<<grid_for_b(B->nnz), 256, 0, B->ctl->stream>>>(B->val, B->val32, B->nnz);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
 template <typename T>
 int launch_batch(sptrsv_batch_s* B, const T* val, const T* d_b, T* d_x, int m, unsigned long long* timer) {
-    sptrsv_plan_s* P = B->plan;
+    sptrsv_plan_s* P = B->ctl;
     BatchArgs<T> a{B->nsys, m, B->sys_ptr, B->lvl_off, B->lvl_ptr, B->rows, B->cfg, B->order,
-                   P->rowptr, P->col, val, d_b, d_x, B->queue, timer};
+                   B->rowptr, B->col, val, d_b, d_x, B->queue, timer};
     k_reset_queue<<<1, 64, 0, P->stream>>>(B->queue);
     k_batch<T><<<B->workgroups, 256, 0, P->stream>>>(a);
     SPT_HIP(hipGetLastError());
@@ -274,8 +314,8 @@ int launch_batch(sptrsv_batch_s* B, const T* val, const T* d_b, T* d_x, int m, u
 }
 
 // The 24 selector features of one system, computed from its CSR and levels (host).
-void system_features(const std::vector<int>& rp, const std::vector<int>& col, const std::vector<double>& val,
-                     const std::vector<int>& lev, int lo, int hi, int m, float* f) {
+void system_features(const int64_t* rp, const int* col, const double* val, const std::vector<int>& lev, int lo, int hi,
+                     int m, float* f) {
     const int n = hi - lo;
     const int64_t e0 = rp[lo], e1 = rp[hi];
     const double nnz = (double)(e1 - e0);
@@ -293,13 +333,13 @@ void system_features(const std::vector<int>& rp, const std::vector<int>& col, co
     double lmax = 0, lsq = 0, diag_only = 0, off_sum = 0, off_max = 0, off1 = 0, off64 = 0, offfar = 0;
     double has_next = 0, rowmax_off = 0, dsum = 0, osum = 0, noff = 0;
     for (int r = lo; r < hi; ++r) {
-        const int len = rp[r + 1] - rp[r];
+        const int64_t len = rp[r + 1] - rp[r];
         lmax = std::max(lmax, (double)len);
         lsq += (double)len * len;
         if (len == 1) diag_only += 1;
         dsum += std::fabs(val[rp[r]]);
         double rmax = 0;
-        for (int p = rp[r] + 1; p < rp[r + 1]; ++p) {
+        for (int64_t p = rp[r] + 1; p < rp[r + 1]; ++p) {
             const double o = col[p] - r;
             off_sum += o;
             off_max = std::max(off_max, o);
@@ -320,9 +360,9 @@ void system_features(const std::vector<int>& rp, const std::vector<int>& col, co
     double in_wide = 0;
     std::vector<int> widths;
     for (int r = lo; r + 1 < hi; ++r) {
-        const int li = rp[r + 1] - rp[r], lj = rp[r + 2] - rp[r + 1];
+        const int64_t li = rp[r + 1] - rp[r], lj = rp[r + 2] - rp[r + 1];
         bool cont = li == lj + 1 && li >= 2 && col[rp[r] + 1] == r + 1;
-        for (int t = 0; cont && t < lj; ++t) cont = col[rp[r] + 1 + t] == col[rp[r + 1] + t];
+        for (int64_t t = 0; cont && t < lj; ++t) cont = col[rp[r] + 1 + t] == col[rp[r + 1] + t];
         if (cont) {
             ++cur;
         } else {
@@ -373,67 +413,107 @@ extern "C" {
 
 int sptrsv_batch_destroy(sptrsv_batch_t B) {
     if (!B) return SPTRSV_OK;
-    if (B->plan && B->plan->stream) (void)hipStreamSynchronize(B->plan->stream);
+    if (B->ctl && B->ctl->stream) (void)hipStreamSynchronize(B->ctl->stream);
     if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
-    void* bufs[] = {B->sys_ptr, B->lvl_off, B->lvl_ptr, B->rows, B->cfg, B->order, B->queue, B->timer};
+    void* bufs[] = {B->rowptr, B->col, B->val, B->val32, B->sys_ptr, B->lvl_off, B->lvl_ptr,
+                    B->rows, B->cfg, B->order, B->queue, B->timer};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    sptrsv_plan_destroy(B->plan);
+    sptrsv_plan_destroy(B->ctl);
     delete B;
     return SPTRSV_OK;
 }
 
 int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t* rowptr, const int* col,
                         const double* val, sptrsv_batch_t* out) {
-    if (!out || !sys_ptr || nsys <= 0) return fail(SPTRSV_ERR_BAD_INPUT, "bad batch arguments");
+    if (!out || !sys_ptr || !rowptr || !col || !val || nsys <= 0) return fail(SPTRSV_ERR_BAD_INPUT, "bad batch arguments");
     *out = nullptr;
     const int n = sys_ptr[nsys];
     if (sys_ptr[0] != 0) return fail(SPTRSV_ERR_BAD_INPUT, "system offsets must start at 0");
     for (int s = 0; s < nsys; ++s)
         if (sys_ptr[s + 1] <= sys_ptr[s]) return fail(SPTRSV_ERR_BAD_INPUT, "empty system in batch");
-    // block-diagonal check: every column of a row stays inside its own system
-    for (int s = 0; s < nsys; ++s)
-        for (int r = sys_ptr[s]; r < sys_ptr[s + 1]; ++r)
-            for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p)
-                if (col[p] >= sys_ptr[s + 1])
-                    return fail(SPTRSV_ERR_BAD_INPUT,
-                                "system " + std::to_string(s) + " references a column of another system");
-    auto* B = new sptrsv_batch_s();
-    int rc = sptrsv_plan_create_csr(n, nnz, rowptr, col, val, &B->plan);
-    if (rc != SPTRSV_OK) {
-        delete B;
-        return rc;
+    if (rowptr[0] != 0 || rowptr[n] != nnz) return fail(SPTRSV_ERR_BAD_INPUT, "rowptr does not span nnz");
+    // validation, per system in parallel: upper CSR with the diagonal first (finite, non-zero),
+    // columns strictly ascending, every column inside the row's own system
+    std::vector<int> bad((size_t)nsys, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s = 0; s < nsys; ++s) {
+        for (int r = sys_ptr[s]; r < sys_ptr[s + 1] && !bad[(size_t)s]; ++r) {
+            const int64_t a = rowptr[r], e = rowptr[r + 1];
+            if (e <= a || col[a] != r || !std::isfinite(val[a]) || val[a] == 0.0) {
+                bad[(size_t)s] = 1;
+                break;
+            }
+            for (int64_t p = a + 1; p < e; ++p)
+                if (col[p] <= col[p - 1] || col[p] >= sys_ptr[s + 1]) {
+                    bad[(size_t)s] = 2;
+                    break;
+                }
+        }
     }
-    B->nsys = nsys;
-    B->h_sys_ptr.assign(sys_ptr, sys_ptr + nsys + 1);
-    // per-system level lists: rows ordered by (system, level, row)
-    std::vector<int> lev((size_t)n);
-    rc = sptrsv_plan_get_levels(B->plan, lev.data());
+    for (int s = 0; s < nsys; ++s) {
+        if (bad[(size_t)s] == 1)
+            return fail(SPTRSV_ERR_BAD_INPUT, "system " + std::to_string(s) +
+                                                  ": a row does not start with a finite non-zero diagonal");
+        if (bad[(size_t)s] == 2)
+            return fail(SPTRSV_ERR_BAD_INPUT, "system " + std::to_string(s) +
+                                                  " has unsorted columns or references a column of another system");
+    }
+    // per-system levels (1-based; rows only reference later rows of their own system)
+    std::vector<int> lev((size_t)n), lcount((size_t)nsys);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s = 0; s < nsys; ++s) {
+        int L = 0;
+        for (int r = sys_ptr[s + 1] - 1; r >= sys_ptr[s]; --r) {
+            int l = 0;
+            for (int64_t p = rowptr[r] + 1; p < rowptr[r + 1]; ++p) l = std::max(l, lev[(size_t)col[p]]);
+            lev[(size_t)r] = l + 1;
+            L = std::max(L, l + 1);
+        }
+        lcount[(size_t)s] = L;
+    }
+    // rows ordered by (system, level, row); lvl_ptr holds, per system, L_s + 1 positions
+    std::vector<int> lvl_off((size_t)nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) lvl_off[(size_t)s + 1] = lvl_off[(size_t)s] + lcount[(size_t)s] + 1;
+    std::vector<int> lvl_ptr((size_t)lvl_off[(size_t)nsys]), rows((size_t)n);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s = 0; s < nsys; ++s) {
+        const int lo = sys_ptr[s], hi = sys_ptr[s + 1], L = lcount[(size_t)s];
+        std::vector<int> cnt((size_t)L + 2, 0);
+        for (int r = lo; r < hi; ++r) cnt[(size_t)lev[(size_t)r] + 1]++;
+        for (int l = 1; l <= L + 1; ++l) cnt[(size_t)l] += cnt[(size_t)l - 1];
+        // cnt[l] = rows with level < l: level l occupies positions lo + [cnt[l], cnt[l+1])
+        for (int l = 1; l <= L + 1; ++l) lvl_ptr[(size_t)lvl_off[(size_t)s] + l - 1] = lo + cnt[(size_t)l];
+        std::vector<int> cur(cnt.begin(), cnt.end());
+        for (int r = lo; r < hi; ++r) rows[(size_t)(lo + cur[(size_t)lev[(size_t)r]]++)] = r;
+    }
+    auto* B = new sptrsv_batch_s();
+    int rc = plan_create_control(n, nnz, &B->ctl);
     if (rc != SPTRSV_OK) {
         sptrsv_batch_destroy(B);
         return rc;
     }
-    std::vector<int> lvl_off(nsys + 1, 0), lvl_ptr, rows((size_t)n);
-    for (int s = 0; s < nsys; ++s) {
-        const int lo = sys_ptr[s], hi = sys_ptr[s + 1];
-        int L = 0;
-        for (int r = lo; r < hi; ++r) L = std::max(L, lev[(size_t)r]);
-        std::vector<int> cnt(L + 2, 0);
-        for (int r = lo; r < hi; ++r) cnt[lev[(size_t)r] + 1]++;
-        for (int l = 1; l <= L + 1; ++l) cnt[l] += cnt[l - 1];
-        // cnt[l] = rows with level < l: level l occupies positions lo + [cnt[l], cnt[l+1])
-        for (int l = 1; l <= L + 1; ++l) lvl_ptr.push_back(lo + cnt[l]);
-        std::vector<int> cur(cnt.begin(), cnt.end());
-        for (int r = lo; r < hi; ++r) rows[(size_t)(lo + cur[lev[(size_t)r]]++)] = r;
-        lvl_off[s + 1] = (int)lvl_ptr.size();
-    }
-    // lvl_ptr holds, per system, L_s + 1 positions: start of level 1 .. end of level L_s
+    B->nsys = nsys;
+    B->n = n;
+    B->nnz = nnz;
+    B->h_sys_ptr.assign(sys_ptr, sys_ptr + nsys + 1);
     std::vector<int> cfg(nsys, SPTRSV_BCFG_LS1), order(nsys);
     for (int s = 0; s < nsys; ++s) order[s] = s;
-    int dev_cus = 0;
-    sptrsv_device_info(nullptr, 0, &dev_cus, nullptr);
-    B->workgroups = std::max(1, dev_cus) * 2;
+    B->workgroups = std::max(1, device_compute_units()) * 2;
+    const size_t e = (size_t)std::max<int64_t>(nnz, 1);
+    if (hipMalloc((void**)&B->rowptr, ((size_t)n + 1) * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&B->col, e * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&B->val, e * sizeof(double)) != hipSuccess) {
+        sptrsv_batch_destroy(B);
+        return fail(SPTRSV_ERR_NOMEM, "device allocation of the batch's matrix");
+    }
+    if (hipMemcpy(B->rowptr, rowptr, ((size_t)n + 1) * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->col, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->val, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        sptrsv_batch_destroy(B);
+        return fail(SPTRSV_ERR_HIP, "upload of the batch's matrix");
+    }
     if ((rc = upload(&B->sys_ptr, B->h_sys_ptr)) || (rc = upload(&B->lvl_off, lvl_off)) ||
         (rc = upload(&B->lvl_ptr, lvl_ptr)) || (rc = upload(&B->rows, rows)) || (rc = upload(&B->cfg, cfg)) ||
         (rc = upload(&B->order, order))) {
@@ -446,15 +526,11 @@ int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t
         sptrsv_batch_destroy(B);
         return fail(SPTRSV_ERR_NOMEM, "batch work buffers");
     }
-    // selector features (host, once)
-    std::vector<int> rp32((size_t)n + 1), c32((size_t)nnz);
-    std::vector<double> v64((size_t)nnz);
-    for (int i = 0; i <= n; ++i) rp32[(size_t)i] = (int)rowptr[i];
-    std::copy(col, col + nnz, c32.begin());
-    std::copy(val, val + nnz, v64.begin());
+    // selector features (host, once, systems in parallel)
     B->h_features.assign((size_t)nsys * 24, 0.f);
+#pragma omp parallel for schedule(dynamic, 4)
     for (int s = 0; s < nsys; ++s)
-        system_features(rp32, c32, v64, lev, sys_ptr[s], sys_ptr[s + 1], 1, &B->h_features[(size_t)s * 24]);
+        system_features(rowptr, col, val, lev, sys_ptr[s], sys_ptr[s + 1], 1, &B->h_features[(size_t)s * 24]);
     *out = B;
     return SPTRSV_OK;
 }
@@ -462,7 +538,9 @@ int sptrsv_batch_create(int nsys, const int* sys_ptr, int64_t nnz, const int64_t
 int sptrsv_batch_info(sptrsv_batch_t B, int* nsys, int* n, int64_t* nnz) {
     if (!B) return fail(SPTRSV_ERR_BAD_INPUT, "null batch");
     if (nsys) *nsys = B->nsys;
-    return sptrsv_plan_info(B->plan, n, nnz, nullptr);
+    if (n) *n = B->n;
+    if (nnz) *nnz = B->nnz;
+    return SPTRSV_OK;
 }
 
 int sptrsv_batch_features(sptrsv_batch_t B, int m, float* feats) {
@@ -498,27 +576,19 @@ int sptrsv_batch_set_configs(sptrsv_batch_t B, const int* cfg, const int* order)
 // use_graph != 0: the launch is captured once per (b, x, m, precision) and replayed.
 int sptrsv_batch_solve(sptrsv_batch_t B, int precision, const double* d_b, double* d_x, int m, int use_graph) {
     if (!B || !d_b || !d_x || m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "bad batch solve arguments");
-    sptrsv_plan_s* P = B->plan;
+    sptrsv_plan_s* P = B->ctl;
     if (precision == SPTRSV_FP32) {
-        SPT_CHECK(ensure_fp32(P));
-        const int64_t nm = (int64_t)P->n * m;
-        if ((int64_t)P->stage_elems < nm) {
-            if (P->b32) SPT_HIP(hipFree(P->b32));
-            if (P->x32) SPT_HIP(hipFree(P->x32));
-            P->b32 = P->x32 = nullptr;
-            SPT_HIP(hipMalloc((void**)&P->b32, nm * sizeof(float)));
-            SPT_HIP(hipMalloc((void**)&P->x32, nm * sizeof(float)));
-            P->stage_elems = (size_t)nm;
-        }
+        SPT_CHECK(ensure_batch_fp32(B));
+        SPT_CHECK(ensure_stage(P, (int64_t)B->n * m));
     } else if (precision != SPTRSV_FP64) {
         return fail(SPTRSV_ERR_BAD_INPUT, "batch precision is FP64 or FP32");
     }
     auto enqueue = [&]() -> int {
-        if (precision == SPTRSV_FP64) return launch_batch<double>(B, P->val, d_b, d_x, m, nullptr);
+        if (precision == SPTRSV_FP64) return launch_batch<double>(B, B->val, d_b, d_x, m, nullptr);
         // FP32 values; b and x are converted on the device around the solve
-        return launch_batch<float>(B, P->val32, P->b32, P->x32, m, nullptr);
+        return launch_batch<float>(B, B->val32, P->b32, P->x32, m, nullptr);
     };
-    if (precision == SPTRSV_FP32) SPT_CHECK(convert_d2f(d_b, P->b32, (int64_t)P->n * m, P->stream));
+    if (precision == SPTRSV_FP32) SPT_CHECK(convert_d2f(d_b, P->b32, (int64_t)B->n * m, P->stream));
     if (!use_graph) {
         hipEvent_t stop;
         SPT_CHECK(begin_timed(P, &stop));
@@ -548,7 +618,7 @@ int sptrsv_batch_solve(sptrsv_batch_t B, int precision, const double* d_b, doubl
         SPT_HIP(hipGraphLaunch(B->gexec, P->stream));
         end_timed(P, stop);
     }
-    if (precision == SPTRSV_FP32) SPT_CHECK(convert_f2d(P->x32, d_x, (int64_t)P->n * m, P->stream));
+    if (precision == SPTRSV_FP32) SPT_CHECK(convert_f2d(P->x32, d_x, (int64_t)B->n * m, P->stream));
     return SPTRSV_OK;
 }
 
@@ -562,17 +632,22 @@ int sptrsv_batch_solve_refined_ex(sptrsv_batch_t B, const sptrsv_refine_t* opts,
     if (opts) ro = *opts;
     if (ro.precision != 0 && ro.precision != SPTRSV_FP32)
         return fail(SPTRSV_ERR_UNSUPPORTED, "batched refinement runs FP32 inner solves");
-    sptrsv_plan_s* P = B->plan;
-    SPT_CHECK(ensure_fp32(P));
+    sptrsv_plan_s* P = B->ctl;
+    SPT_CHECK(ensure_batch_fp32(B));
     auto solve_low = [&](const float* in, float* out) -> int {
-        return launch_batch<float>(B, P->val32, in, out, m, nullptr);
+        return launch_batch<float>(B, B->val32, in, out, m, nullptr);
     };
-    auto residual = [&](const double* b, const double* x, float* r) -> int { return residual32(P, b, x, r, m); };
+    auto residual = [&](const double* b, const double* x, float* r) -> int {
+        k_batch_residual<float><<<grid_for_b((int64_t)B->n * m), 256, 0, P->stream>>>(B->n, m, B->rowptr, B->col,
+                                                                                      B->val, x, b, r);
+        SPT_HIP(hipGetLastError());
+        return SPTRSV_OK;
+    };
     RefineResult res{};
     SPT_CHECK(refine_loop(P, ro, d_b, d_x, m, solve_low, residual, &res));
     int fell_back = 0;
     if (res.outcome != SPTRSV_REFINE_CONVERGED && ro.fallback) {
-        SPT_CHECK(launch_batch<double>(B, P->val, d_b, d_x, m, nullptr));
+        SPT_CHECK(launch_batch<double>(B, B->val, d_b, d_x, m, nullptr));
         SPT_CHECK(sptrsv_plan_sync(P));
         fell_back = 1;
     }
@@ -597,25 +672,32 @@ int sptrsv_batch_solve_refined(sptrsv_batch_t B, const sptrsv_refine_t* opts, co
 
 int sptrsv_batch_sync(sptrsv_batch_t B) {
     if (!B) return fail(SPTRSV_ERR_BAD_INPUT, "null batch");
-    return sptrsv_plan_sync(B->plan);
+    return sptrsv_plan_sync(B->ctl);
 }
 
-sptrsv_plan_t sptrsv_batch_plan(sptrsv_batch_t B) { return B ? B->plan : nullptr; }
+// r = b - U x over every system, FP64, asynchronous on the batch's stream.
+int sptrsv_batch_residual(sptrsv_batch_t B, const double* d_b, const double* d_x, double* d_r, int m) {
+    if (!B || !d_b || !d_x || !d_r || m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "bad residual arguments");
+    k_batch_residual<double><<<grid_for_b((int64_t)B->n * m), 256, 0, B->ctl->stream>>>(B->n, m, B->rowptr, B->col,
+                                                                                        B->val, d_x, d_b, d_r);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
 
 // Autotuner primitive: one launch with every system on configuration `cfg` (systems for which
 // cfg is invalid -- CHAIN with m > 32 -- run LS_G1), device time of each system in microseconds.
 int sptrsv_batch_time_config(sptrsv_batch_t B, int cfg, int precision, const double* d_b, double* d_x, int m,
                              double* sys_us) {
     if (!B || !sys_us || cfg < 0 || cfg >= SPTRSV_BCFG_COUNT) return fail(SPTRSV_ERR_BAD_INPUT, "bad timing arguments");
-    sptrsv_plan_s* P = B->plan;
+    sptrsv_plan_s* P = B->ctl;
     std::vector<int> all((size_t)B->nsys, cfg);
     SPT_HIP(hipMemcpy(B->cfg, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
     int rc = SPTRSV_OK;
     if (precision == SPTRSV_FP64) {
-        rc = launch_batch<double>(B, P->val, d_b, d_x, m, B->timer);
+        rc = launch_batch<double>(B, B->val, d_b, d_x, m, B->timer);
     } else {
-        rc = sptrsv_batch_solve(B, SPTRSV_FP32, d_b, d_x, m, 0);  // allocate staging, warm up
-        if (rc == SPTRSV_OK) rc = launch_batch<float>(B, P->val32, P->b32, P->x32, m, B->timer);
+        rc = sptrsv_batch_solve(B, SPTRSV_FP32, d_b, d_x, m, 0);  // FP32 copy and staging, warm up
+        if (rc == SPTRSV_OK) rc = launch_batch<float>(B, B->val32, P->b32, P->x32, m, B->timer);
     }
     if (rc == SPTRSV_OK) rc = sptrsv_plan_sync(P);
     std::vector<unsigned long long> t(2 * (size_t)B->nsys);

@@ -76,6 +76,8 @@ struct sptrsv_plan_s {
     std::vector<int> h_snodes;
     std::vector<int> h_sn_r;        // rectangular column count per supernode
     int64_t* sn_offr16 = nullptr;   // offsets of the BF16 blocks (rows padded to 16 columns)
+    int64_t sn_padded = 0;          // explicit zeros stored by the layout (relaxed supernodes)
+    int relaxed_state = 0;          // AUTO's relaxed supernodes: 0 not tried, 1 built, -1 not used
 
     // refinement work buffers (n*m)
     double* ref_r = nullptr;
@@ -129,6 +131,7 @@ struct sptrsv_plan_s {
     bool borrowed_stream = false;  // stream owned by the plan this one is the companion of
 
     hipStream_t stream = nullptr;
+    int device = 0;  // HIP device the plan's memory and stream live on
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     size_t events_used = 0;
@@ -159,6 +162,8 @@ int default_variant(sptrsv_plan_s* p, int m, int precision, bool exact);  // sol
 int solve_fp32_device(sptrsv_plan_s* p, const sptrsv_params_t* params, const float* d_b32, float* d_x32,
                       int m);  // solve.hip
 int ensure_stage(sptrsv_plan_s* p, int64_t nm);  // solve.hip: FP32 staging buffers b32 / x32
+bool relaxed_supernodes_pay(sptrsv_plan_s* p);  // supernodal.hip: builds the relaxed layout once
+int plan_create_control(int n, int64_t nnz, sptrsv_plan_s** out);  // runtime.hip: no factor arrays
 int supernodal_solve_low(sptrsv_plan_s* p, int precision, const float* d_b32, float* d_x32, int m,
                          uint64_t limit);  // supernodal.hip
 int tile_solve(sptrsv_plan_s* p, int precision, int rows, const double* d_b, double* d_x, int m,

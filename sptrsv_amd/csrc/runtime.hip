@@ -66,6 +66,21 @@ void end_timed(sptrsv_plan_s* p, hipEvent_t stop) {
     if (stop) (void)hipEventRecord(stop, p->stream);
 }
 
+// A plan that holds no factor: stream, status word, timing events and the refinement / FP32
+// staging buffers, for solvers that keep their own matrix arrays (the batched solver).
+int plan_create_control(int n, int64_t nnz, sptrsv_plan_s** out) {
+    SPT_CHECK(ensure_device());
+    auto* p = new sptrsv_plan_s();
+    p->n = n;
+    p->nnz = nnz;
+    *out = p;
+    SPT_CHECK(sptrsv_malloc((void**)&p->ticket, 64 * sizeof(int)));
+    SPT_HIP(hipMemset(p->ticket, 0, 64 * sizeof(int)));
+    SPT_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    SPT_HIP(hipGetDevice(&p->device));
+    p->ticks_per_ms = wallclock_ticks_per_ms();
+    return SPTRSV_OK;
+}
 }  // namespace sptrsv
 
 using namespace sptrsv;
@@ -212,6 +227,7 @@ static int plan_alloc_common(sptrsv_plan_s* p) {
     SPT_CHECK(sptrsv_malloc((void**)&p->lval, nnz * sizeof(double)));
     SPT_CHECK(sptrsv_malloc((void**)&p->ticket, 64 * sizeof(int)));
     SPT_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    SPT_HIP(hipGetDevice(&p->device));
     p->ticks_per_ms = wallclock_ticks_per_ms();
     return SPTRSV_OK;
 }

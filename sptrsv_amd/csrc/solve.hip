@@ -323,6 +323,7 @@ __global__ void k_reset_ticket(int* ctrl) {
     if (threadIdx.x == 0) ctrl[0] = 0;
 }
 
+
 template <typename Tin, typename Tout>
 __global__ void k_convert(const Tin* __restrict__ in, Tout* __restrict__ out, int64_t count) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -636,7 +637,9 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // its per-step work outgrows the level-ordered pull kernel's); everything else to LEVEL_PULL.
 // With `exact`, only kernels whose FP64 x is bitwise equal to the serial solver qualify.
 int default_variant(sptrsv_plan_s* P, int m, int precision, bool exact) {
-    if (!exact && m == 1 && (int64_t)P->n_levels * 4 > (int64_t)P->n) return SPTRSV_VAR_CHAIN;
+    const bool chain_like = (int64_t)P->n_levels * 4 > (int64_t)P->n;
+    if (!exact && chain_like && precision == SPTRSV_FP64 && relaxed_supernodes_pay(P)) return SPTRSV_VAR_SUPERNODAL;
+    if (!exact && m == 1 && chain_like) return SPTRSV_VAR_CHAIN;
     if (m <= 3 && precision == SPTRSV_FP64 && P->grid && block_supports(P, m)) return SPTRSV_VAR_BLOCK;
     return SPTRSV_VAR_LEVEL_PULL;
 }

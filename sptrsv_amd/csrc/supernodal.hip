@@ -745,39 +745,56 @@ int supernodal_solve_low(sptrsv_plan_s* P, int precision, const float* d_b32, fl
 
 using namespace sptrsv;
 
-extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const int* snodes_in) {
-    if (!P) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
-    const int n = P->n;
-    const int64_t nnz = P->nnz;
-    std::vector<int> rp((size_t)n + 1), col((size_t)nnz);
-    std::vector<double> val((size_t)nnz);
-    SPT_HIP(hipStreamSynchronize(P->stream));
-    SPT_HIP(hipMemcpy(rp.data(), P->rowptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
-    SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
-    SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+namespace sptrsv {
+namespace {
 
-    // ---- boundaries ----------------------------------------------------------------------
-    std::vector<int> bounds;
-    if (sn_in > 0) {
-        if (!snodes_in) return fail(SPTRSV_ERR_BAD_INPUT, "null supernode array");
-        if (snodes_in[0] != 0 || snodes_in[sn_in] != n)
-            return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must start at 0 and end at n");
-        for (int k = 0; k < sn_in; ++k)
-            if (snodes_in[k + 1] <= snodes_in[k])
-                return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must increase");
-        bounds.assign(snodes_in, snodes_in + sn_in + 1);
-    } else {
-        // fundamental supernodes: row i+1 continues row i's supernode when
-        // pattern(i) == {i} U pattern(i+1)
-        bounds.push_back(0);
-        for (int i = 0; i + 1 < n; ++i) {
-            const int li = rp[i + 1] - rp[i], lj = rp[i + 2] - rp[i + 1];
-            bool cont = li == lj + 1 && li >= 2 && col[rp[i] + 1] == i + 1;
-            for (int t = 0; cont && t < lj; ++t) cont = col[rp[i] + 1 + t] == col[rp[i + 1] + t];
-            if (!cont) bounds.push_back(i + 1);
+// Relaxed supernodes: consecutive rows are merged while the merged block, padded with explicit
+// zeros to a dense triangle plus one common column set (the union of the rows' columns right of
+// the block), holds at most (1 + max_fill) times the rows' own entries and at most max_width
+// rows.  A band of width 100 (the exact Cholesky factor of a 2D Laplacian) merges into blocks of
+// 128 rows with w(w-1)/2 explicit zeros each.
+std::vector<int> relaxed_bounds(int n, const std::vector<int>& rp, const std::vector<int>& col, int max_width,
+                                double max_fill) {
+    // block [s0, s0 + w): cnt[c] = block rows referencing column c; uni = distinct referenced
+    // columns right of the block (>= s0 + w); orig = the rows' own entries
+    std::vector<int> bounds{0}, cnt((size_t)n, 0), touched;
+    int s0 = 0, uni = 0;
+    int64_t orig = 0;
+    for (int i = 0; i < n;) {
+        const int64_t w = i - s0;
+        // row i would join: column i moves inside the block, row i's columns (all > i) join the
+        // union unless already referenced
+        int uni2 = uni - (cnt[(size_t)i] > 0 ? 1 : 0);
+        for (int p = rp[i] + 1; p < rp[i + 1]; ++p) uni2 += cnt[(size_t)col[p]] == 0 ? 1 : 0;
+        const int64_t orig2 = orig + (rp[i + 1] - rp[i]);
+        const int64_t padded = (w + 1) * (w + 2) / 2 + (w + 1) * (int64_t)uni2;
+        if (w > 0 && (w + 1 > max_width || (double)(padded - orig2) > max_fill * (double)orig2)) {
+            bounds.push_back(i);  // close the block before row i and start a new one at it
+            for (int c : touched) cnt[(size_t)c] = 0;
+            touched.clear();
+            s0 = i;
+            uni = 0;
+            orig = 0;
+            continue;
         }
-        bounds.push_back(n);
+        for (int p = rp[i] + 1; p < rp[i + 1]; ++p)
+            if (cnt[(size_t)col[p]]++ == 0) touched.push_back(col[p]);
+        uni = uni2;
+        orig = orig2;
+        ++i;
     }
+    bounds.push_back(n);
+    return bounds;
+}
+
+}  // namespace
+
+// Supernodal layout from boundaries.  exact: every supernode must have the structure of the
+// reference's supernodes (a dense triangle followed by one common column set in every row,
+// src/core.cpp:53-78); otherwise rows are padded with explicit zeros to that structure.
+int build_sn_layout(sptrsv_plan_s* P, const std::vector<int>& rp, const std::vector<int>& col,
+                    const std::vector<double>& val, const std::vector<int>& bounds, bool exact) {
+    const int n = P->n;
     // split wide supernodes (the first part's extra columns join its rectangular set)
     std::vector<int> sb;
     for (size_t k = 0; k + 1 < bounds.size(); ++k)
@@ -792,22 +809,35 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
     std::vector<int64_t> offt(S), offr(S);
     int64_t tot_t = 0, tot_r = 0;
     int wmax = 0;
+    std::vector<int> mark((size_t)n, -1), uni;
     for (int k = 0; k < S; ++k) {
         const int s0 = sb[k], w = sb[k + 1] - s0;
         wmax = std::max(wmax, w);
-        const int len0 = rp[s0 + 1] - rp[s0];
-        bool ok = len0 >= w;
-        for (int t = 0; ok && t < w; ++t) ok = col[rp[s0] + t] == s0 + t;
-        const int r = ok ? len0 - w : 0;
-        for (int d = 1; ok && d < w; ++d) {
-            const int row = s0 + d, a = rp[row];
-            ok = rp[row + 1] - a == (w - d) + r;
-            for (int t = 0; ok && t < w - d; ++t) ok = col[a + t] == row + t;
-            for (int q = 0; ok && q < r; ++q) ok = col[a + (w - d) + q] == col[rp[s0] + w + q];
+        if (exact) {
+            const int len0 = rp[s0 + 1] - rp[s0];
+            bool ok = len0 >= w;
+            for (int t = 0; ok && t < w; ++t) ok = col[rp[s0] + t] == s0 + t;
+            const int r = ok ? len0 - w : 0;
+            for (int d = 1; ok && d < w; ++d) {
+                const int row = s0 + d, a = rp[row];
+                ok = rp[row + 1] - a == (w - d) + r;
+                for (int t = 0; ok && t < w - d; ++t) ok = col[a + t] == row + t;
+                for (int q = 0; ok && q < r; ++q) ok = col[a + (w - d) + q] == col[rp[s0] + w + q];
+            }
+            if (!ok)
+                return fail(SPTRSV_ERR_BAD_INPUT, "rows " + std::to_string(s0) + ".." + std::to_string(s0 + w - 1) +
+                                                      " do not form a supernode (dense triangle + common columns)");
         }
-        if (!ok)
-            return fail(SPTRSV_ERR_BAD_INPUT, "rows " + std::to_string(s0) + ".." + std::to_string(s0 + w - 1) +
-                                                  " do not form a supernode (dense triangle + common columns)");
+        // rectangular columns: the union of the rows' columns right of the block, ascending
+        uni.clear();
+        for (int d = 0; d < w; ++d)
+            for (int p = rp[s0 + d]; p < rp[s0 + d + 1]; ++p)
+                if (col[p] >= s0 + w && mark[(size_t)col[p]] != k) {
+                    mark[(size_t)col[p]] = k;
+                    uni.push_back(col[p]);
+                }
+        std::sort(uni.begin(), uni.end());
+        const int r = (int)uni.size();
         rcount[k] = r;
         offt[k] = tot_t;
         offr[k] = tot_r;
@@ -815,8 +845,7 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
         tot_t += (int64_t)w * w;
         tot_r += (int64_t)w * r;
         int last = -1;
-        for (int q = 0; q < r; ++q) {
-            const int c = col[rp[s0] + w + q];
+        for (int c : uni) {
             cols.push_back(c);
             if (sn_of[(size_t)c] != last) {
                 last = sn_of[(size_t)c];
@@ -825,13 +854,20 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
         }
         dep_ptr[k + 1] = (int)dep.size();
     }
-    std::vector<double> T((size_t)tot_t, 0.0), R((size_t)tot_r);
+    std::vector<double> T((size_t)tot_t, 0.0), R((size_t)tot_r, 0.0);
     for (int k = 0; k < S; ++k) {
         const int s0 = sb[k], w = sb[k + 1] - s0, r = rcount[k];
+        const int* cb = cols.data() + offc[k];
         for (int d = 0; d < w; ++d) {
-            const int a = rp[s0 + d];
-            for (int e = d; e < w; ++e) T[(size_t)(offt[k] + (int64_t)d * w + e)] = val[(size_t)(a + e - d)];
-            for (int q = 0; q < r; ++q) R[(size_t)(offr[k] + (int64_t)d * r + q)] = val[(size_t)(a + (w - d) + q)];
+            for (int p = rp[s0 + d]; p < rp[s0 + d + 1]; ++p) {
+                const int c = col[p];
+                if (c < s0 + w) {
+                    T[(size_t)(offt[k] + (int64_t)d * w + (c - s0))] = val[(size_t)p];
+                } else {
+                    const int q = (int)(std::lower_bound(cb, cb + r, c) - cb);
+                    R[(size_t)(offr[k] + (int64_t)d * r + q)] = val[(size_t)p];
+                }
+            }
         }
     }
 
@@ -861,6 +897,90 @@ extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const in
     P->sn_epoch = 0;
     P->sn_flag_blocks = 1;
     P->h_snodes = sb;
+    int64_t stored = tot_t;
+    for (int k = 0; k < S; ++k) stored -= (int64_t)(sb[k + 1] - sb[k]) * (sb[k + 1] - sb[k] - 1) / 2;
+    P->sn_padded = stored + tot_r - P->nnz;
+    return SPTRSV_OK;
+}
+
+int download_csr(sptrsv_plan_s* P, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val) {
+    rp.resize((size_t)P->n + 1);
+    col.resize((size_t)P->nnz);
+    val.resize((size_t)P->nnz);
+    SPT_HIP(hipStreamSynchronize(P->stream));
+    SPT_HIP(hipMemcpy(rp.data(), P->rowptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
+    SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return SPTRSV_OK;
+}
+
+// AUTO on a chain-like factor: relaxed supernodes of at least 32 rows on average turn its
+// dependency chain of n rows into one of n / w blocks, each solved on FP64 MFMA.  Built once.
+bool relaxed_supernodes_pay(sptrsv_plan_s* P) {
+    if (P->relaxed_state == 0) {
+        P->relaxed_state = -1;
+        if ((double)P->nnz / std::max(P->n, 1) >= 16.0 && P->sn == 0) {
+            std::vector<int> rp, col;
+            std::vector<double> val;
+            if (download_csr(P, rp, col, val) == SPTRSV_OK) {
+                const std::vector<int> b = relaxed_bounds(P->n, rp, col, kMaxW, 1.0);
+                if ((double)P->n / (double)(b.size() - 1) >= 32.0 &&
+                    build_sn_layout(P, rp, col, val, b, false) == SPTRSV_OK)
+                    P->relaxed_state = 1;
+            }
+        }
+    }
+    return P->relaxed_state == 1 && P->sn > 0;
+}
+
+}  // namespace sptrsv
+
+extern "C" int sptrsv_plan_build_supernodes(sptrsv_plan_t P, int sn_in, const int* snodes_in) {
+    if (!P) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
+    const int n = P->n;
+    std::vector<int> rp, col;
+    std::vector<double> val;
+    SPT_CHECK(download_csr(P, rp, col, val));
+    std::vector<int> bounds;
+    if (sn_in > 0) {
+        if (!snodes_in) return fail(SPTRSV_ERR_BAD_INPUT, "null supernode array");
+        if (snodes_in[0] != 0 || snodes_in[sn_in] != n)
+            return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must start at 0 and end at n");
+        for (int k = 0; k < sn_in; ++k)
+            if (snodes_in[k + 1] <= snodes_in[k])
+                return fail(SPTRSV_ERR_BAD_INPUT, "supernode boundaries must increase");
+        bounds.assign(snodes_in, snodes_in + sn_in + 1);
+    } else {
+        // fundamental supernodes: row i+1 continues row i's supernode when
+        // pattern(i) == {i} U pattern(i+1)
+        bounds.push_back(0);
+        for (int i = 0; i + 1 < n; ++i) {
+            const int li = rp[i + 1] - rp[i], lj = rp[i + 2] - rp[i + 1];
+            bool cont = li == lj + 1 && li >= 2 && col[rp[i] + 1] == i + 1;
+            for (int t = 0; cont && t < lj; ++t) cont = col[rp[i] + 1 + t] == col[rp[i + 1] + t];
+            if (!cont) bounds.push_back(i + 1);
+        }
+        bounds.push_back(n);
+    }
+    P->relaxed_state = -1;  // an explicit layout replaces any relaxed one AUTO may build
+    return build_sn_layout(P, rp, col, val, bounds, true);
+}
+
+extern "C" int sptrsv_plan_build_supernodes_relaxed(sptrsv_plan_t P, int max_width, double max_fill) {
+    if (!P) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
+    if (max_width < 0 || max_fill < 0) return fail(SPTRSV_ERR_BAD_INPUT, "negative relaxation limit");
+    std::vector<int> rp, col;
+    std::vector<double> val;
+    SPT_CHECK(download_csr(P, rp, col, val));
+    const int w = max_width == 0 ? kMaxW : std::min(max_width, kMaxW);
+    const std::vector<int> bounds = relaxed_bounds(P->n, rp, col, w, max_fill == 0 ? 1.0 : max_fill);
+    P->relaxed_state = -1;
+    return build_sn_layout(P, rp, col, val, bounds, false);
+}
+
+extern "C" int sptrsv_plan_supernodal_padding(sptrsv_plan_t P, int64_t* explicit_zeros) {
+    if (!P || !explicit_zeros) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    *explicit_zeros = P->sn > 0 ? P->sn_padded : 0;
     return SPTRSV_OK;
 }
 

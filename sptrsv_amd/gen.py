@@ -223,6 +223,34 @@ def random_supernodal(S: int = 4096, seed: int = 2, wmin: int = 16, wmax: int = 
     return UpperCSR(n, rowptr, col, val, start.astype(np.int32), f"random_supernodal_{S}")
 
 
+def scale_diagonal(U: UpperCSR, s: float) -> UpperCSR:
+    """A copy of U with every diagonal entry multiplied by s (the diagonal is first in each row):
+    s < 1 weakens the diagonal dominance and raises the condition number, s > 1 lowers it."""
+    val = U.val.copy()
+    val[U.rowptr[:-1]] *= s
+    return UpperCSR(U.n, U.rowptr, U.col, val, U.snodes, f"{U.name}_d{s:.6g}")
+
+
+# Diagonal scales that put kappa_1(U) at 1e2 ... 1e8 (one per decade) for the two families of the
+# mixed-precision sweep, the random Zipf factor (n = 1e6, seed 1) and the random supernodal factor
+# (4,096 supernodes, seed 2).  Found by tests/golden/calibrate_kappa.py (Hager/Higham estimate
+# on the CPU); the GPU run re-estimates kappa for every scaled factor and reports that value.
+KAPPA_SCALES: dict = {
+    "zipf_1e6": {1e2: 346.196, 1e3: 99.8524, 1e4: 32.0, 1e5: 9.92624, 1e6: 3.3663, 1e7: 1.29564,
+                 1e8: 0.608024},
+    # estimated kappa_1: 98.8, 996, 9.32e3, 9.96e4, 9.88e5, 9.6e6, 9.89e7
+    "supernodal_4096": {1e2: 1135.85, 1e3: 322.128, 1e4: 80.017, 1e5: 26.5364, 1e6: 9.01103, 1e7: 3.26189,
+                        1e8: 1.45479},
+    # estimated kappa_1: 99.2, 995, 9.98e3, 1.09e5, 9.97e5, 9.12e6, 9.46e7
+}
+
+
+def kappa_scaled(family: str, target: float) -> UpperCSR:
+    """The family's factor with the diagonal scale calibrated for condition number `target`."""
+    base = random_zipf(1_000_000, seed=1) if family == "zipf_1e6" else random_supernodal(4096, seed=2)
+    return scale_diagonal(base, KAPPA_SCALES[family][float(target)])
+
+
 def banded(n: int, bw: int, seed: int = 0) -> UpperCSR:
     """Random banded upper factor: every row holds all columns i..min(i+bw, n-1); off-diagonal
     values U(-1,1), diagonal 1 + sum |offdiag| of its row (diagonally dominant)."""
@@ -274,6 +302,33 @@ def batch_systems(count: int = 4096, seed: int = 3, nmin: int = 256, nmax: int =
     val = np.concatenate([p.val for p in parts])
     total = UpperCSR(int(sys_ptr[-1]), rowptr, col, val, None, f"batch_{count}_{seed}")
     return total, sys_ptr.astype(np.int32), kinds
+
+
+def batch_systems_native(count: int = 4096, seed: int = 3, nmin: int = 256, nmax: int = 65536, first: int = 0):
+    """The batched-solve workload at full size, drawn by the library's native generator
+    (include/sptrsv_gen.h): banded (bandwidth U{4..64}), random Zipf (Poisson(8) rows, Zipf(1.5)
+    offsets) and random supernodal systems shaped like random_supernodal's defaults (widths
+    U[16,128], 1-6 ancestors), n ~ log-uniform[nmin, nmax].  Returns (U, sys_ptr int32[count+1],
+    kinds list) like batch_systems; U.rowptr is int64 (the batch may pass 2^31 nonzeros).
+    first: index of the first system, so that chunks reproduce pieces of one larger draw."""
+    from . import capi
+    L = capi.lib()
+    ns = np.zeros(count, dtype=np.int32)
+    nz = np.zeros(count, dtype=np.int64)
+    kd = np.zeros(count, dtype=np.int32)
+    capi.check(L.sptrsv_gen_batch_sizes(count, first, seed, nmin, nmax, capi._ptr(ns), capi._ptr(nz), capi._ptr(kd)),
+               "gen_batch_sizes")
+    n = int(ns.sum())
+    nnz = int(nz.sum())
+    rowptr = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    capi.check(L.sptrsv_gen_batch_fill(count, first, seed, nmin, nmax, capi._ptr(ns), capi._ptr(nz), capi._ptr(rowptr),
+                                       capi._ptr(col), capi._ptr(val)), "gen_batch_fill")
+    sys_ptr = np.zeros(count + 1, dtype=np.int32)
+    np.cumsum(ns, out=sys_ptr[1:])
+    kinds = [BATCH_KINDS[k] for k in kd]
+    return UpperCSR(n, rowptr, col, val, None, f"batch_native_{first}_{count}_{seed}"), sys_ptr, kinds
 
 
 def rhs_for(U: UpperCSR, m: int = 1, seed: int = 0, layout: str = "col"):

@@ -82,6 +82,26 @@ class Selector:
     def close(self):
         self.mlp.close()
 
+    def save(self, path: str, meta: dict | None = None):
+        """Network sizes, input normalisation and parameters as JSON text."""
+        import json
+        doc = {"sizes": list(self.mlp.sizes), "mu": self.mu.tolist(), "sd": self.sd.tolist(),
+               "params": self.mlp.get_params().tolist(), "meta": meta or {}}
+        with open(path, "w") as f:
+            json.dump(doc, f)
+
+    @classmethod
+    def load(cls, path: str) -> "Selector":
+        import json
+        with open(path) as f:
+            doc = json.load(f)
+        sel = cls(hidden=int(doc["sizes"][1]))
+        sel.mu = np.asarray(doc["mu"], dtype=np.float32)
+        sel.sd = np.asarray(doc["sd"], dtype=np.float32)
+        sel.mlp.set_params(np.asarray(doc["params"], dtype=np.float32))
+        sel.meta = doc.get("meta", {})
+        return sel
+
 
 def sub_batch(U, sys_ptr, idx):
     """systems `idx` of a block-diagonal batch as a batch of their own: (CSR, sys_ptr)"""
@@ -127,3 +147,75 @@ def regret(times: np.ndarray, choice: np.ndarray) -> float:
     """Sum of chosen device times over the sum of the best ones, minus one."""
     chosen = times[np.arange(len(choice)), choice]
     return float(chosen.sum() / times.min(axis=1).sum() - 1.0)
+
+
+def collect(total: int, chunk: int, seed: int, nmin: int, nmax: int, ms=(1, 4, 16), reps: int = 2, log=print):
+    """Autotuner labels for `total` systems of the native generator (seed, drawn in chunks that
+    reproduce one draw): per m, the features (total x 24) and the exhaustive timing table
+    (total x K).  Right-hand sides are random: the timings do not depend on their values."""
+    from . import gen
+    feats = {m: [] for m in ms}
+    times = {m: [] for m in ms}
+    rng = np.random.default_rng(seed)
+    for first in range(0, total, chunk):
+        cnt = min(chunk, total - first)
+        U, sp, _ = gen.batch_systems_native(cnt, seed=seed, nmin=nmin, nmax=nmax, first=first)
+        B = capi.Batch(U, sp)
+        n, nnz = U.n, U.nnz
+        del U
+        for m in ms:
+            b = rng.uniform(-1.0, 1.0, (n, m) if m > 1 else n)
+            d_b, d_x = capi.DeviceArray.from_host(b), capi.DeviceArray(b.shape)
+            del b
+            try:
+                times[m].append(exhaustive(B, d_b, d_x, m=m, reps=reps))
+            finally:
+                d_b.free()
+                d_x.free()
+            feats[m].append(B.features(m))
+        B.close()
+        log(f"[tune] systems {first}..{first + cnt - 1}: n={n} nnz={nnz} timed")
+    return {m: (np.concatenate(feats[m]), np.concatenate(times[m])) for m in ms}
+
+
+def train_main(argv=None):
+    """python -m sptrsv_amd.tune --count 20000 --chunk 2000 --seed 4 --out sptrsv_amd/selector.json
+
+    Collects the autotuner's labels on the native generator's systems and trains the selector
+    network on the GPU; writes its parameters and the training summary as JSON."""
+    import argparse
+    import json
+    import os
+    import sys
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--count", type=int, default=20000)
+    ap.add_argument("--chunk", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--nmin", type=int, default=256)
+    ap.add_argument("--nmax", type=int, default=65536)
+    ap.add_argument("--epochs", type=int, default=1500)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector.json"))
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    ms = (1, 4, 16)
+    data = collect(a.count, a.chunk, a.seed, a.nmin, a.nmax, ms, a.reps, log)
+    sel = Selector(seed=0)
+    loss = sel.fit_multi([data[m][0] for m in ms], [data[m][1] for m in ms], list(ms), epochs=a.epochs)
+    summary = {"count": a.count, "seed": a.seed, "nmin": a.nmin, "nmax": a.nmax, "ms": list(ms),
+               "epochs": a.epochs, "final_loss": loss, "wall_s": round(time.time() - t0, 1)}
+    for m in ms:
+        F, T = data[m]
+        pick = sel.predict(F, m)
+        best = np.argmin(T, axis=1)
+        summary[f"m{m}"] = {"train_accuracy": float(np.mean(pick == best)), "train_regret": regret(T, pick),
+                            "label_counts": np.bincount(best, minlength=K).tolist()}
+    sel.save(a.out, summary)
+    print(json.dumps(summary))
+    sel.close()
+
+
+if __name__ == "__main__":
+    train_main()

@@ -146,3 +146,99 @@ def test_sub_batch_extracts_systems():
         assert np.array_equal(np.asarray(U.col)[e0:e1] - r0, np.asarray(S.col)[f0:f1] - q0)
         assert np.array_equal(np.asarray(U.val)[e0:e1], np.asarray(S.val)[f0:f1])
         assert np.array_equal(np.diff(rp[r0:r1 + 1]), np.diff(np.asarray(S.rowptr)[q0:q1 + 1]))
+
+
+# ---- native batch generator (include/sptrsv_gen.h) -----------------------------------------------
+
+def _native(count=60, seed=7, nmin=256, nmax=4000):
+    return gen.batch_systems_native(count, seed=seed, nmin=nmin, nmax=nmax)
+
+
+def test_native_batch_is_valid_block_diagonal_and_reproducible():
+    from sptrsv_amd import capi
+    U, sp, kinds = _native()
+    assert capi.validate_csr(U.n, U.rowptr, U.col, U.val) == capi.OK
+    rows = np.repeat(np.arange(U.n), np.diff(U.rowptr))
+    sys_of = np.searchsorted(sp, np.arange(U.n), side="right") - 1
+    assert np.array_equal(sys_of[U.col], sys_of[rows])  # no entry leaves its system
+    U2, sp2, k2 = _native()
+    assert np.array_equal(U.val, U2.val) and np.array_equal(U.col, U2.col) and kinds == k2
+    U3, _, _ = _native(seed=8)
+    assert U3.nnz != U.nnz or not np.array_equal(U3.val, U.val)
+    # chunks with `first` reproduce pieces of the whole draw
+    Ua, spa, ka = _native(count=25)
+    Ub, spb, kb = gen.batch_systems_native(35, seed=7, nmin=256, nmax=4000, first=25)
+    assert ka + kb == kinds
+    assert np.array_equal(np.concatenate([Ua.val, Ub.val]), U.val)
+    assert np.array_equal(np.concatenate([Ua.col, Ub.col + Ua.n]), U.col)
+    ns = np.diff(sp)
+    assert set(kinds) == {"banded", "zipf", "supernodal"}
+    for s, k in enumerate(kinds):
+        if k != "supernodal":
+            assert 256 <= ns[s] <= 4000
+
+
+def test_native_banded_and_zipf_systems():
+    U, sp, kinds = _native(count=90)
+    rp, col, val = U.rowptr, U.col, U.val
+    offs_all = []
+    for s, k in enumerate(kinds):
+        if k == "supernodal":
+            continue
+        lo, hi = sp[s], sp[s + 1]
+        seg = slice(rp[lo], rp[hi])
+        rows = np.repeat(np.arange(lo, hi), np.diff(rp[lo:hi + 1]))
+        c, v = col[seg], val[seg]
+        diag = c == rows
+        absum = np.bincount(rows[~diag] - lo, weights=np.abs(v[~diag]), minlength=hi - lo)
+        assert np.allclose(v[diag], 1.0 + absum, rtol=0, atol=1e-12), k  # diagonal 1 + sum |off|
+        assert np.all(np.abs(v[~diag]) <= 1.0)
+        if k == "banded":
+            bw = int(np.diff(rp[lo:hi + 1]).max()) - 1
+            assert 4 <= bw <= 64
+            n = hi - lo
+            assert np.array_equal(np.diff(rp[lo:hi + 1]), np.minimum(bw + 1, n - np.arange(n)))
+        elif k == "zipf":
+            offs_all.append(c[~diag] - rows[~diag])
+    offs = np.concatenate(offs_all)
+    # the same rows drawn with numpy's Poisson and Zipf samplers (random_zipf): after the cap and
+    # the removal of duplicates, the shares of small offsets and the row length must agree
+    R = gen.random_zipf(200_000, seed=5)
+    rrows = np.repeat(np.arange(R.n), np.diff(R.rowptr))
+    roffs = (R.col - rrows)[R.col != rrows]
+    for k in (1, 2, 3, 5, 10):
+        assert abs(np.mean(offs == k) - np.mean(roffs == k)) < 0.01, k
+    band = lambda o: np.mean((o > 20) & (o <= 100))  # tails beyond ~n are capped differently
+    assert abs(band(offs) - band(roffs)) < 0.01
+    zn = sum(sp[s + 1] - sp[s] for s, k in enumerate(kinds) if k == "zipf")
+    assert abs(len(offs) / zn - len(roffs) / R.n) < 0.1
+
+
+def test_native_supernodal_systems():
+    U, sp, kinds = _native(count=60, nmin=2000, nmax=6000)
+    rp, col, val = U.rowptr, U.col, U.val
+    seen = 0
+    offd = []
+    for s, k in enumerate(kinds):
+        if k != "supernodal":
+            continue
+        seen += 1
+        lo, hi = sp[s], sp[s + 1]
+        # rows of one supernode: dense triangle then the same rectangular columns
+        r = lo
+        while r < hi:
+            c0 = col[rp[r]:rp[r + 1]]
+            w = 1
+            while r + w < hi and np.array_equal(col[rp[r + w]:rp[r + w + 1]], c0[w:]) and c0[w] == r + w:
+                w += 1
+            # a supernode whose columns continue exactly into its neighbour reads as one wider
+            # run here, so only the lower bound is a property of every run
+            assert w >= 16 or r + w == hi
+            assert np.array_equal(c0[:w], np.arange(r, r + w))
+            d = val[rp[r]]
+            assert 1.0 <= d < 2.0
+            offd.append(val[rp[r] + 1:rp[r + 1]])
+            r += w
+    assert seen > 5
+    o = np.concatenate(offd)
+    assert abs(o.mean()) < 0.01 and abs(o.std() - 0.1) < 0.01

@@ -64,6 +64,18 @@ def test_c1_solve(c1, variant):
     assert O.rel_error(xt, ref) < 1e-9
 
 
+def test_c1_relaxed_supernodes(c1):
+    """AUTO turns the band into relaxed supernodes of 128 rows: 79 dense blocks on MFMA."""
+    U, P = c1
+    assert P.auto_variant(1) == capi.VAR_SUPERNODAL
+    sb = P.supernodes()
+    assert len(sb) - 1 == (U.n + 127) // 128 and np.all(np.diff(sb)[:-1] == 128)
+    for m in (1, 8, 32):
+        xt, b = gen.rhs_for(U, m, seed=0, layout="row")
+        x = P.solve_host(b, capi.params())
+        assert O.rel_error(oracle_rows(U, b), x) <= FP64_TOL, m
+
+
 def test_c1_exact_auto_is_bitwise(c1):
     U, P = c1
     xt, b = gen.rhs_for(U, 1, seed=0)

@@ -181,6 +181,18 @@ int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* vari
  * Asynchronous on the plan's stream. */
 int sptrsv_solve(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
                  double* d_x, int m);
+/* No-allocation variant of sptrsv_solve with SPTRSV_VAR_CSC_PUSH, FP64.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The push kernel's work memory
+ * (left_sum n*m doubles, consumed in-degrees n ints, ticket and status words) lives in a fixed
+ * array inside the library, one per GPU, held until the process ends; the call itself allocates
+ * nothing.  Limits: n <= SPTRSV_NOALLOC_MAX_ROWS and n*m <= SPTRSV_NOALLOC_MAX_ELEMS, else
+ * SPTRSV_ERR_UNSUPPORTED with nothing launched.  Every call on one GPU shares the array: calls
+ * that overlap on one GPU race (keep them in one stream order).  Asynchronous on the plan's
+ * stream; sptrsv_plan_sync reports a watchdog stop as for sptrsv_solve. */
+#define SPTRSV_NOALLOC_MAX_ROWS (1 << 21)
+#define SPTRSV_NOALLOC_MAX_ELEMS (1 << 21)
+int sptrsv_solve_push_noalloc(sptrsv_plan_t plan, const double* d_b, double* d_x, int m);
 /* Waits for the plan's stream; returns SPTRSV_ERR_DEADLOCK if a watchdog fired. */
 int sptrsv_plan_sync(sptrsv_plan_t plan);
 /* Host pointers, synchronous.  col_major != 0: b and x are column-major (x[i + k*n]). */

@@ -21,6 +21,8 @@ ERR_IO = -4
 ERR_NOMEM = -5
 ERR_UNSUPPORTED = -6
 ERR_NO_DEVICE = -7
+NOALLOC_MAX_ROWS = 1 << 21
+NOALLOC_MAX_ELEMS = 1 << 21
 
 VAR_AUTO = 0
 VAR_CSC_PUSH = 1
@@ -155,6 +157,7 @@ def lib():
         "sptrsv_batch_solve_refined_ex": (_i, [_vp, ctypes.POINTER(RefineOpts), _vp, _vp, _i,
                                                ctypes.POINTER(RefineInfo)]),
         "sptrsv_condest": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "sptrsv_solve_push_noalloc": (_i, [_PLAN, _vp, _vp, _i]),
         "sptrsv_gen_batch_sizes": (_i, [_i, _i, ctypes.c_uint64, _i, _i, _vp, _vp, _vp]),
         "sptrsv_gen_batch_fill": (_i, [_i, _i, ctypes.c_uint64, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     }
@@ -430,6 +433,10 @@ class Plan:
 
     def sync(self):
         check(lib().sptrsv_plan_sync(self.handle), "sync")
+
+    def solve_push_noalloc(self, d_b: DeviceArray, d_x: DeviceArray, m: int = 1):
+        """Asynchronous CSC push solve with the library's fixed work array (no allocation)."""
+        check(lib().sptrsv_solve_push_noalloc(self.handle, d_b.ptr, d_x.ptr, int(m)), "solve_push_noalloc")
 
     def solve_host(self, b: np.ndarray, prm: Params | None = None, col_major: bool = False):
         """Synchronous solve of host b: shape (n,), (n, m) row-major, or (m, n) with col_major."""

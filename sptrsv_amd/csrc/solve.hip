@@ -21,11 +21,22 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "common.h"
 #include "plan.h"
+
+// Work memory of sptrsv_solve_push_noalloc: one fixed array per GPU, part of the code object's
+// own device memory and held until the process ends.  Layout: bytes [0, 256) ticket and status
+// words, then SPTRSV_NOALLOC_MAX_ROWS consumed in-degrees (int), then SPTRSV_NOALLOC_MAX_ELEMS
+// left_sum values (double) -- 24 MiB in all.  Host code never uses this name as an address: the
+// per-device address comes from hipGetSymbolAddress (noalloc_scratch below).
+constexpr size_t kNoAllocIndegOff = 256;
+constexpr size_t kNoAllocSumOff = kNoAllocIndegOff + (size_t)SPTRSV_NOALLOC_MAX_ROWS * sizeof(int);
+constexpr size_t kNoAllocBytes = kNoAllocSumOff + (size_t)SPTRSV_NOALLOC_MAX_ELEMS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 namespace sptrsv {
 namespace {
@@ -323,6 +334,15 @@ __global__ void k_reset_ticket(int* ctrl) {
     if (threadIdx.x == 0) ctrl[0] = 0;
 }
 
+// ticket and status of a solve whose control words are not the plan's own
+__global__ void k_reset_ctrl(int* ctrl) {
+    if (threadIdx.x < 2) ctrl[threadIdx.x] = 0;
+}
+
+// a watchdog that fired in `ctrl` is reported through the plan's status word (sptrsv_plan_sync)
+__global__ void k_carry_status(const int* ctrl, int* plan_ctrl) {
+    if (threadIdx.x == 0 && ctrl[1] != 0) plan_ctrl[1] = ctrl[1];
+}
 
 template <typename Tin, typename Tout>
 __global__ void k_convert(const Tin* __restrict__ in, Tout* __restrict__ out, int64_t count) {
@@ -675,6 +695,32 @@ int solve_fp32_device(sptrsv_plan_s* P, const sptrsv_params_t* prm, const float*
                               P->cval32, P->lval32, limit);
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on device `dev`, looked up once per device with that device
+// current (each GPU holds its own copy of the array at its own address).
+int noalloc_scratch(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(SPTRSV_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        SPT_HIP(hipGetDevice(&cur));
+        if (cur != dev) SPT_HIP(hipSetDevice(dev));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return fail(SPTRSV_ERR_HIP, std::string("hipGetSymbolAddress(scratch) -> ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return SPTRSV_OK;
+}
+
 }  // namespace sptrsv
 
 using namespace sptrsv;
@@ -718,6 +764,36 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
         return SPTRSV_OK;
     }
     return fail(SPTRSV_ERR_BAD_INPUT, "unknown precision");
+}
+
+// The CSC push solve (k_push) with its work memory in the library's fixed per-GPU array: no
+// allocation.  Every call on a GPU shares that array, and each call clears what must start at
+// zero (left_sum, the control words) and copies the in-degrees in, all with kernels on the
+// plan's stream: calls that overlap on one GPU race, calls in one stream order do not.
+int sptrsv_solve_push_noalloc(sptrsv_plan_t P, const double* d_b, double* d_x, int m) {
+    if (!P || !d_b || !d_x) return fail(SPTRSV_ERR_BAD_INPUT, "null argument");
+    if (m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "m must be >= 1");
+    const int64_t n = P->n, nm = n * m;
+    if (n > SPTRSV_NOALLOC_MAX_ROWS || nm > SPTRSV_NOALLOC_MAX_ELEMS)
+        return fail(SPTRSV_ERR_UNSUPPORTED, "no-allocation solve: n = " + std::to_string(n) + ", n*m = " +
+                                                std::to_string(nm) + " exceed the fixed work array (n <= " +
+                                                std::to_string(SPTRSV_NOALLOC_MAX_ROWS) + ", n*m <= " +
+                                                std::to_string(SPTRSV_NOALLOC_MAX_ELEMS) + ")");
+    unsigned char* base = nullptr;
+    SPT_CHECK(noalloc_scratch(P->device, &base));
+    int* ctrl = reinterpret_cast<int*>(base);
+    int* indeg = reinterpret_cast<int*>(base + kNoAllocIndegOff);
+    double* ls = reinterpret_cast<double*>(base + kNoAllocSumOff);
+    hipStream_t s = P->stream;
+    const uint64_t limit = P->ticks_per_ms * 20000ull;
+    k_fill_bits<double><<<grid_for(nm), 256, 0, s>>>(ls, 0, nm);
+    k_copy_int<<<grid_for(n), 256, 0, s>>>(P->indeg, indeg, n);
+    k_reset_ctrl<<<1, 64, 0, s>>>(ctrl);
+    PushArgs<double> a{P->n, m, P->colptr, P->rowidx, P->cval, indeg, ls, d_b, d_x, ctrl, limit};
+    k_push<double><<<P->n, 64, 0, s>>>(a);
+    k_carry_status<<<1, 64, 0, s>>>(ctrl, P->ticket);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
 }
 
 int sptrsv_solve_host(sptrsv_plan_t P, const sptrsv_params_t* params, const double* b, double* x, int m,

@@ -59,7 +59,17 @@ def supernodal_flops(U, m):
     return float(np.sum((2 * w * r + w * w) * m))
 
 
+_WORKLOADS = {}
+
+
 def make_workload(name):
+    """(factor, description), generated once per process"""
+    if name not in _WORKLOADS:
+        _WORKLOADS[name] = _make_workload(name)
+    return _WORKLOADS[name]
+
+
+def _make_workload(name):
     from sptrsv_amd import gen
     if name == "ic0_128":
         return gen.laplace3d_ic0(128), "IC(0) of 7-point Laplacian, 128^3 grid"
@@ -104,29 +114,69 @@ def read_traffic(workload, variant, m):
         return None
 
 
-def cpu_baseline(U, m, budget_s):
-    """The oracle's serial solve (a restatement of the reference's base_gauss_upper) on one
-    host core, repeated until `budget_s` seconds of solve time or 50 solves."""
-    from oracle import oracle as O
-    from sptrsv_amd import gen
-    xt, b = gen.rhs_for(U, m, seed=1, layout="col")
-    times = []
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < budget_s and len(times) < 50:
-        _, t = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
-        times.append(t)
-    best = min(times)
-    _, ta, ts = O.barrier_upper(U.rowptr, U.col, U.val, b, O.max_threads())
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "")}
+
+
+def cpu_baseline(workloads, headline, m):
+    """The reference's CPU methods restated in the oracle, timed on the host in 3 fresh processes
+    (python -m oracle.cpu_baseline, like run.py:24-32), minimum over the processes.  The headline
+    figure is the serial solve (one core) of the bench workload; the OpenMP level-set solve (all
+    threads) and, on the supernodal factor, the supernodal and CBLAS supernodal solves ride along."""
+    import subprocess
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="sptrsv_cpu_")
+    names = []
+    for name, U in workloads.items():
+        arrays = {"rowptr": U.rowptr, "col": U.col, "val": U.val}
+        if U.snodes is not None:
+            arrays["snodes"] = U.snodes
+        np.savez(os.path.join(tmp, name + ".npz"), **arrays)
+        names.append(name)
+    runs = []
+    for _ in range(3):
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", tmp, *names], capture_output=True, text=True,
+                           cwd=ROOT, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu baseline process failed: {r.stderr[-2000:]}")
+        runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    for name in names:
+        os.remove(os.path.join(tmp, name + ".npz"))
+    os.rmdir(tmp)
+    best = {}
+    for name in names:
+        rec = dict(runs[0][name])
+        for key in rec:
+            if key.endswith("_ms"):
+                rec[key] = round(min(r[name][key] for r in runs), 4)
+        best[name] = rec
+    h = best[headline]
+    U = workloads[headline]
     return {
-        "value": alg_bytes(U.n, U.nnz, m) / best / 1e9,
+        "value": alg_bytes(U.n, U.nnz, m) / (h["serial_ms"] / 1e3) / 1e9,
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{len(times)} full serial solves of the same factor (m={m}), min time {best * 1e3:.2f} ms",
-        "serial_ms": best * 1e3,
-        "openmp_levelset_ms": (ta + ts) * 1e3,
-        "openmp_levelset_analysis_ms": ta * 1e3,
-        "openmp_threads": O.max_threads(),
+        "sample": (f"serial solve of the same factor (m={m}) repeated within 2 s per process, "
+                   f"minimum over 3 fresh processes; {h['serial_ms']:.3f} ms"),
+        "serial_ms": h["serial_ms"],
+        "openmp_levelset_ms": h["levelset_ms"],
+        "openmp_threads": h["levelset_threads"],
+        "host": host_info(),
+        "workloads": best,
     }
 
 
@@ -343,7 +393,6 @@ def main():
     ap.add_argument("--variant", default="block")
     ap.add_argument("--m", type=int, default=1)
     ap.add_argument("--occupancy", type=int, default=0, help="params.occupancy of the measured solve")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -491,9 +540,17 @@ def main():
             "ms": round(min(th) * 1e3, 4), "GB_s": round(B / min(th) / 1e9, 2),
             "note": "sptrsv_solve_host: b host->device, solve, x device->host (plan resident)"}
         progress(f"host-buffer solve: {extras[f'{args.workload}/{args.variant}/m{m}/host_buffers']}")
-    cpu = None if args.no_cpu else cpu_baseline(U, m, args.cpu_budget)
-    if cpu:
-        progress(f"cpu baseline: {cpu['sample']}")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            wl = {args.workload: U}
+            if m == 1 and not args.no_extras:
+                for extra in ("chol2d_100", "zipf_1e6", "supernodal_4096"):
+                    wl[extra] = make_workload(extra)[0]
+            cpu = cpu_baseline(wl, args.workload, m)
+            progress(f"cpu baseline: {cpu['workloads']}")
+        except Exception as exc:  # report, never hide
+            cpu = {"error": str(exc)}
     del P
     mixed, batch, sweep = {}, {}, {}
     if not args.no_mixed:

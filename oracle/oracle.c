@@ -10,6 +10,7 @@
  */
 #include "oracle.h"
 
+#include <dlfcn.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -405,6 +406,51 @@ double orc_supernodal_blas_upper(int n, int sn, const int* snodes, double* x, co
         }
     }
     double t = omp_get_wtime() - t0;
+    free(g);
+    free(c);
+    return t;
+}
+
+/* src/core.cpp:164-201 (supernodal_blas_upper) with the reference's two BLAS calls made
+ * through a CBLAS library loaded at run time (dlopen of `blas_path`: MKL's libmkl_rt where the
+ * host has it, as the reference links MKL, else an OpenBLAS).  Same call sequence and
+ * arguments as src/core.cpp:186-194: cblas_dgemm(RowMajor, NoTrans, NoTrans, w, 1, r, 1, R,
+ * lda, g, 1, 1, c, 1), x -= c, cblas_dtrsm(RowMajor, Left, Upper, NoTrans, NonUnit, w, 1, 1,
+ * T, lda, x, 1).  The CBLAS enumerators are the standard ones (RowMajor 101, NoTrans 111,
+ * Upper 121, NonUnit 131, Left 141).  Returns the solve seconds, or -1 if the library or its
+ * symbols cannot be loaded. */
+typedef void (*orc_dgemm_t)(int, int, int, int, int, int, double, const double*, int, const double*, int, double,
+                            double*, int);
+typedef void (*orc_dtrsm_t)(int, int, int, int, int, int, int, double, const double*, int, double*, int);
+
+double orc_supernodal_blas_upper_cblas(const char* blas_path, int n, int sn, const int* snodes, double* x,
+                                       const double* val_pad, const uint64_t* col_pad, const int* rowptr_pad,
+                                       int rhs) {
+    void* h = dlopen(blas_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1.0;
+    orc_dgemm_t gemm = (orc_dgemm_t)dlsym(h, "cblas_dgemm");
+    orc_dtrsm_t trsm = (orc_dtrsm_t)dlsym(h, "cblas_dtrsm");
+    if (!gemm || !trsm) return -1.0;
+    double* g = (double*)calloc((size_t)n, sizeof(double));
+    double* c = (double*)calloc((size_t)n, sizeof(double));
+    const double t0 = omp_get_wtime();
+    for (int rh = 0; rh < rhs; ++rh) {
+        const int64_t off = (int64_t)rh * n;
+        for (int k = sn - 1; k >= 0; --k) {
+            const int s = snodes[k], w = snodes[k + 1] - snodes[k];
+            const int64_t head = rowptr_pad[s];
+            const int lda = (int)((rowptr_pad[s + w] - rowptr_pad[s]) / w);
+            const int r = (int)(rowptr_pad[s + 1] - rowptr_pad[s] - w);
+            for (int q = 0; q < r; ++q) g[q] = x[off + (int64_t)col_pad[head + w + q]];
+            gemm(101, 111, 111, w, 1, r, 1.0, val_pad + head + w, lda, g, 1, 1.0, c, 1);
+            for (int d = 0; d < w; ++d) {
+                x[off + s + d] -= c[d];
+                c[d] = 0.0;
+            }
+            trsm(101, 141, 121, 111, 131, w, 1, 1.0, val_pad + head, lda, x + off + s, 1);
+        }
+    }
+    const double t = omp_get_wtime() - t0;
     free(g);
     free(c);
     return t;

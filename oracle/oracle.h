@@ -78,6 +78,11 @@ void orc_spmv_residual(int n, const double* val, const uint64_t* rowptr, const i
 double orc_wtime(void);
 int orc_max_threads(void);
 
+/* src/core.cpp:164-201 with cblas_dgemm / cblas_dtrsm from a BLAS loaded at run time. */
+double orc_supernodal_blas_upper_cblas(const char* blas_path, int n, int sn, const int* snodes, double* x,
+                                       const double* val_pad, const uint64_t* col_pad, const int* rowptr_pad,
+                                       int rhs);
+
 #ifdef __cplusplus
 }
 #endif

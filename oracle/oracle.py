@@ -61,6 +61,8 @@ def lib():
                                          ctypes.POINTER(_c_i64)]),
             "orc_supernodal_blas_upper": (_c_dbl, [_c_int, _c_int, _i32p, _dp, _dp, _u64p, _i32p, _c_int]),
             "orc_rel_error": (_c_dbl, [_c_i64, _dp, _dp]),
+            "orc_supernodal_blas_upper_cblas": (_c_dbl, [ctypes.c_char_p, _c_int, _c_int, _i32p, _dp, _dp, _u64p,
+                                                         _i32p, _c_int]),
             "orc_spmv_residual": (None, [_c_int, _dp, _u64p, _i32p, _dp, _dp, _dp, _c_int]),
             "orc_wtime": (_c_dbl, []),
             "orc_max_threads": (_c_int, []),
@@ -227,6 +229,38 @@ def supernodal_blas_upper(val_pad, col_pad, rowptr_pad, snodes, b):
     t = lib().orc_supernodal_blas_upper(n, len(sn_arr) - 1, sn_arr, x,
                                         np.ascontiguousarray(val_pad, dtype=np.float64),
                                         np.ascontiguousarray(col_pad, dtype=np.uint64), rpp, x.shape[0])
+    return x.reshape(np.shape(b)), t
+
+
+CBLAS_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1", "libmkl_rt.so",
+                    "libopenblas.so", "libopenblas.so.0", "libcblas.so", "libcblas.so.3")
+
+
+def find_cblas():
+    """First CBLAS library (exporting cblas_dgemm and cblas_dtrsm) that loads here, or None."""
+    for path in CBLAS_CANDIDATES:
+        try:
+            h = ctypes.CDLL(path)
+            getattr(h, "cblas_dgemm")
+            getattr(h, "cblas_dtrsm")
+            return path
+        except (OSError, AttributeError):
+            continue
+    return None
+
+
+def supernodal_blas_upper_cblas(blas_path, val_pad, col_pad, rowptr_pad, snodes, b):
+    """supernodal_blas_upper (src/core.cpp:164-201) with the BLAS calls made through the CBLAS
+    library at blas_path.  In place on a copy of b; returns (x, seconds)."""
+    sn_arr = np.ascontiguousarray(snodes, dtype=np.int32)
+    rpp = np.ascontiguousarray(rowptr_pad, dtype=np.int32)
+    n = len(rpp) - 1
+    x = _rhs(b, n).copy()
+    t = lib().orc_supernodal_blas_upper_cblas(os.fsencode(blas_path), n, len(sn_arr) - 1, sn_arr, x,
+                                              np.ascontiguousarray(val_pad, dtype=np.float64),
+                                              np.ascontiguousarray(col_pad, dtype=np.uint64), rpp, x.shape[0])
+    if t < 0:
+        raise OSError(f"cannot load cblas_dgemm / cblas_dtrsm from {blas_path}")
     return x.reshape(np.shape(b)), t
 
 

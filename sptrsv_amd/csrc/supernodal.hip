@@ -101,9 +101,9 @@ constexpr int kSnLdt = kMaxW + 1;  // LDS row stride (doubles) of a 16-row block
 // RESID: the FP64 residual of the refinement, r_s = B_s - T_s X_s - R_s X(rect) (no waits; the
 // triangle is one more K segment with columns s0 .. s0 + w - 1), stored as FP32.
 template <int NW, bool RESID>
-__global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve(SnArgs a) {
+__global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : NW == 8 ? 1 : 8 / NW) void k_sn_solve(SnArgs a) {
     constexpr int NT = 64 * NW;
-    constexpr int kSnKC = NW == 4 ? 32 : 16;  // K chunk (register budget: 2 waves per SIMD)
+    constexpr int kSnKC = NW >= 4 ? 32 : 16;  // K chunk (register budget: 2 waves per SIMD)
     constexpr int kSnLdr = kSnKC + 1;         // LDS row stride (doubles) of a K chunk of R
     constexpr int kSnBuf = 2 * kMaxW * kSnLdr;
     static_assert(kSnBuf >= 2 * 16 * kSnLdt, "Tinv blocks must fit the R double buffer");
@@ -672,13 +672,17 @@ void supernodal_free(sptrsv_plan_s* p) {
     p->h_snodes.clear();
 }
 
+// Waves per workgroup for nblk 16-column blocks of X: above 4 blocks one workgroup of 8 waves
+// takes all of them (up to 128 columns), so each supernode's R is streamed from HBM once.
+inline int sn_waves(int nblk) { return nblk > 4 ? 8 : nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1; }
+
 int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double* d_x, int m, uint64_t limit) {
     if (P->sn == 0)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
     if (precision != SPTRSV_FP64)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant: FP64 only through sptrsv_solve");
     const int nblk = (m + 15) / 16;
-    const int nw = nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1;
+    const int nw = sn_waves(nblk);
     const int npanel = (nblk + nw - 1) / nw;
     SPT_CHECK(ensure_sn_flags(P, npanel));
     P->sn_epoch++;
@@ -689,7 +693,8 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
     const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
-    if (nw == 4) k_sn_solve<4, false><<<grid, 256, 0, P->stream>>>(a);
+    if (nw == 8) k_sn_solve<8, false><<<grid, 512, 0, P->stream>>>(a);
+    else if (nw == 4) k_sn_solve<4, false><<<grid, 256, 0, P->stream>>>(a);
     else if (nw == 2) k_sn_solve<2, false><<<grid, 128, 0, P->stream>>>(a);
     else k_sn_solve<1, false><<<grid, 64, 0, P->stream>>>(a);
     end_timed(P, stop);
@@ -702,13 +707,14 @@ int supernodal_residual32(sptrsv_plan_s* P, const double* d_b, const double* d_x
     if (P->sn == 0)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant needs sptrsv_plan_build_supernodes first");
     const int nblk = (m + 15) / 16;
-    const int nw = nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1;
+    const int nw = sn_waves(nblk);
     const int npanel = (nblk + nw - 1) / nw;
     SnArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
              P->sn_rect, P->sn_tri, P->sn_dep_ptr, P->sn_dep, P->sn_flag, 0, d_b, const_cast<double*>(d_x), d_r32,
              P->ticket, 0};
     const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
-    if (nw == 4) k_sn_solve<4, true><<<grid, 256, 0, P->stream>>>(a);
+    if (nw == 8) k_sn_solve<8, true><<<grid, 512, 0, P->stream>>>(a);
+    else if (nw == 4) k_sn_solve<4, true><<<grid, 256, 0, P->stream>>>(a);
     else if (nw == 2) k_sn_solve<2, true><<<grid, 128, 0, P->stream>>>(a);
     else k_sn_solve<1, true><<<grid, 64, 0, P->stream>>>(a);
     SPT_HIP(hipGetLastError());

@@ -266,3 +266,22 @@ def test_planted_mistake_is_caught():
         x[i] = (1.0 - acc) / val[rp[i]]
     want = [float(Fraction(s)) for s in EXPECTED["csr_6_upper"]["x"]]
     assert not np.allclose(x, want)
+
+
+def test_supernodal_blas_cblas_matches_loops():
+    """The CBLAS-calling restatement of supernodal_blas_upper (the reference's two BLAS calls
+    through a run-time-loaded CBLAS) against the plain-loop restatement and the serial solve."""
+    path = O.find_cblas()
+    if path is None:
+        pytest.skip("no CBLAS library on this host")
+    U = gen.random_supernodal(40, seed=9)
+    vp, cp, rpp = O.ccs2ccs_pad(U.rowptr, U.col, U.val, U.snodes)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, (3, U.n))
+    xb, t = O.supernodal_blas_upper_cblas(path, vp, cp, rpp, U.snodes, b)
+    xl, _ = O.supernodal_blas_upper(vp, cp, rpp, U.snodes, b)
+    xs, _ = O.base_gauss_upper(U.rowptr, U.col, U.val, b)
+    assert t >= 0
+    assert O.rel_error(xl, xb) <= 1e-13 and O.rel_error(xs, xb) <= 1e-12
+    with pytest.raises(OSError):
+        O.supernodal_blas_upper_cblas("/nonexistent/libcblas.so", vp, cp, rpp, U.snodes, b)

@@ -61,6 +61,7 @@ def run(path, name):
             out["blas_ms"], out["blas_reps"] = _best(
                 lambda: O.supernodal_blas_upper_cblas(lib, vp, cp, rpp, sn, b)[1], 1.0, 5)
             out["blas_library"] = lib
+            out["blas_threading"] = os.environ.get("MKL_THREADING_LAYER", "")
         else:  # no CBLAS on the host: the plain-loop restatement of the same method
             out["blas_ms"], out["blas_reps"] = _best(lambda: O.supernodal_blas_upper(vp, cp, rpp, sn, b)[1], 1.0, 5)
             out["blas_library"] = "none (plain loops)"

@@ -426,8 +426,13 @@ typedef void (*orc_dtrsm_t)(int, int, int, int, int, int, int, double, const dou
 double orc_supernodal_blas_upper_cblas(const char* blas_path, int n, int sn, const int* snodes, double* x,
                                        const double* val_pad, const uint64_t* col_pad, const int* rowptr_pad,
                                        int rhs) {
+    /* MKL's runtime would start its own OpenMP runtime next to the one already in the process
+     * and abort; the calls are matrix-vector sized, so its sequential layer is the right one */
+    setenv("MKL_THREADING_LAYER", "SEQUENTIAL", 0);
     void* h = dlopen(blas_path, RTLD_NOW | RTLD_LOCAL);
     if (!h) return -1.0;
+    int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+    if (set_layer) set_layer(1); /* MKL_THREADING_SEQUENTIAL */
     orc_dgemm_t gemm = (orc_dgemm_t)dlsym(h, "cblas_dgemm");
     orc_dtrsm_t trsm = (orc_dtrsm_t)dlsym(h, "cblas_dtrsm");
     if (!gemm || !trsm) return -1.0;

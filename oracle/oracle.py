@@ -238,6 +238,7 @@ CBLAS_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.
 
 def find_cblas():
     """First CBLAS library (exporting cblas_dgemm and cblas_dtrsm) that loads here, or None."""
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")  # see orc_supernodal_blas_upper_cblas
     for path in CBLAS_CANDIDATES:
         try:
             h = ctypes.CDLL(path)

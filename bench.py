@@ -183,7 +183,8 @@ def cpu_baseline(workloads, headline, m):
 def bench_mixed(capi, gen, names, out):
     """Wall time of the refined solve (reduced-precision solves + FP64 residual and update on the
     GPU, stopped at an estimated relative error <= 1e-13) against the plain FP64 solve of the
-    same kernel family, data resident on the GPU; rel_err_vs_fp64 is measured, not estimated."""
+    same kernel family and (m <= 32) the plain FP64 sync-free CSC push solve, data resident on
+    the GPU; rel_err_vs_fp64 is measured, not estimated."""
     from sptrsv_amd import gen as G
     cases = [("ic0_128", "level_pull", 32, capi.FP32), ("ic0_128", "level_pull", 128, capi.FP32),
              ("zipf_1e6", "chain", 1, capi.FP32), ("supernodal_4096", "supernodal", 128, capi.BF16),
@@ -213,6 +214,17 @@ def bench_mixed(capi, gen, names, out):
                 P.sync()
                 t64.append(time.perf_counter() - t0)
             x64 = d_x.download()
+            tpush = None
+            if m <= 32:
+                # the plain FP64 sync-free kernel (CSC push, the reference's algorithm) on the same
+                # system: the refined solve's other point of comparison
+                prm_push = capi.params(variant=capi.VAR_CSC_PUSH)
+                P.solve(d_b, d_r, m, prm_push)
+                P.sync()
+                t0 = time.perf_counter()
+                P.solve(d_b, d_r, m, prm_push)
+                P.sync()
+                tpush = time.perf_counter() - t0
             P.solve_refined(d_b, d_r, m, prm, precision=prec)  # warm-up (builds reduced copies)
             tref = []
             for _ in range(3):
@@ -224,6 +236,9 @@ def bench_mixed(capi, gen, names, out):
             out[key] = {"fp64_ms": round(min(t64) * 1e3, 3), "refined_ms": round(min(tref) * 1e3, 3),
                         "speedup": round(min(t64) / min(tref), 3), "iters": it, "error_estimate": ratio,
                         "rel_err_vs_fp64": err, "low_precision": "bf16+fp32" if prec == capi.BF16 else "fp32"}
+            if tpush is not None:
+                out[key]["fp64_syncfree_ms"] = round(tpush * 1e3, 3)
+                out[key]["speedup_vs_syncfree"] = round(tpush / min(tref), 3)
             for d in (d_b, d_x, d_r):
                 d.free()
         except Exception as exc:
@@ -234,18 +249,19 @@ def bench_mixed(capi, gen, names, out):
 KAPPA_TARGETS = (1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8)
 
 
-def bench_kappa(capi, gen, out, plan_ms):
+def bench_kappa(capi, gen, out, plan_ms, targets=None):
     """Mixed precision with FP64 refinement across the condition-number sweep: the Zipf factor
     (n = 1e6) and the supernodal factor (4,096 supernodes) with the diagonal scaled so that
     kappa_1 spans 1e2 ... 1e8 (gen.KAPPA_SCALES).  Per factor: kappa re-estimated on the GPU
     (sptrsv_condest); per m: the plain FP64 solve, the refined solve (FP32 inner solves for Zipf,
     BF16 rectangular products for the supernodal factor) with the FP64 fallback on, wall times
-    with data resident, and the refined x against the serial oracle.  plan_ms: family -> m list."""
+    with data resident, and the refined x against the serial oracle.  plan_ms: family -> m list;
+    targets: the kappa values (all of KAPPA_TARGETS by default)."""
     from oracle import oracle as O
     for fam, ms in plan_ms.items():
         base = gen.random_zipf(1_000_000, seed=1) if fam == "zipf_1e6" else gen.random_supernodal(4096, seed=2)
         sn = fam.startswith("supernodal")
-        for tgt in KAPPA_TARGETS:
+        for tgt in (targets or KAPPA_TARGETS):
             key = f"{fam}/kappa{tgt:.0e}"
             res = {"diag_scale": gen.KAPPA_SCALES[fam][tgt]}
             try:
@@ -301,7 +317,7 @@ def bench_kappa(capi, gen, out, plan_ms):
 SELECTOR_PATH = os.path.join(ROOT, "sptrsv_amd", "selector.json")
 
 
-def bench_batch(capi, gen, out, count=4096, nmax=65536, ms=(1, 4, 16)):
+def bench_batch(capi, gen, out, count=4096, nmax=65536, ms=(1,)):
     """Batched solve of `count` systems from the native generator (seed 3; n log-uniform
     [256, nmax]; banded / Zipf / supernodal shaped like the 4,096-supernode factor) in one
     persistent graph-replayed launch, for each m: the plain fallback configuration (LS1),
@@ -398,7 +414,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run N-byte device copies (counter calibration)")
-    ap.add_argument("--suite", default="", help="kappa: only the full condition-number sweep (m in 1, 32, 128)")
+    ap.add_argument("--suite", default="", help="kappa: only the full condition-number sweep (m in 1, 32, 128); "
+                    "batch: only the batched solve at m in 1, 4, 16")
     ap.add_argument("--no-kappa", action="store_true")
     args = ap.parse_args()
     t_start = time.perf_counter()
@@ -417,6 +434,11 @@ def main():
         sweep = {}
         bench_kappa(capi, gen, sweep, {"zipf_1e6": (1, 32, 128), "supernodal_4096": (1, 32, 128)})
         print(json.dumps({"suite": "kappa", "kappa_sweep": sweep, "bench_wall_s": round(time.perf_counter() - t_start, 1)}))
+        return
+    if args.suite == "batch":
+        batch = {}
+        bench_batch(capi, gen, batch, ms=(1, 4, 16))
+        print(json.dumps({"suite": "batch", "batched": batch, "bench_wall_s": round(time.perf_counter() - t_start, 1)}))
         return
 
     names = {v: k for k, v in capi.VARIANT_NAMES.items()}
@@ -446,7 +468,8 @@ def main():
     if not args.no_extras:
         runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
                 ("ic0_128", "level_pull", 32), ("ic0_128", "level_pull", 128), ("zipf_1e6", "tile", 1),
-                ("zipf_1e6", "chain", 1), ("chol2d_100", "chain", 1), ("chol2d_100", "level_pull", 1),
+                ("zipf_1e6", "chain", 1), ("zipf_1e6", "auto", 1), ("chol2d_100", "chain", 1),
+                ("chol2d_100", "auto", 1), ("chol2d_100", "level_pull", 1),
                 ("supernodal_4096", "supernodal", 8), ("supernodal_4096", "supernodal", 32),
                 ("supernodal_4096", "supernodal", 128)]
         cache = {args.workload: (U, P)}
@@ -556,7 +579,7 @@ def main():
     if not args.no_mixed:
         bench_mixed(capi, gen, names, mixed)
     if not args.no_kappa:
-        bench_kappa(capi, gen, sweep, {"zipf_1e6": (1,), "supernodal_4096": (32,)})
+        bench_kappa(capi, gen, sweep, {"zipf_1e6": (1,), "supernodal_4096": (32,)}, targets=(1e2, 1e8))
     if not args.no_batch:
         try:
             bench_batch(capi, gen, batch)

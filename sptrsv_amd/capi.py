@@ -33,7 +33,7 @@ VAR_CHAIN = 5
 VAR_SUPERNODAL = 6
 VAR_TILE = 7
 VAR_BLOCK = 8
-VARIANT_NAMES = {VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
+VARIANT_NAMES = {VAR_AUTO: "auto", VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
                  VAR_LEVELSET: "levelset", VAR_CHAIN: "chain",
                  VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile", VAR_BLOCK: "block"}
 

@@ -122,6 +122,13 @@ struct sptrsv_plan_s {
     int* blk_step = nullptr;     // per position: 1 + the largest step of its in-block dependencies
     double* blk_val = nullptr;
 
+    // chain variant: one record per 32-row window (packed inverse of its triangle, near and old
+    // entries), built on first use; FP64 and FP32 values
+    char* chain_rec = nullptr;
+    char* chain_rec32 = nullptr;
+    int chain_rec_bytes = 0, chain_rec32_bytes = 0;
+    int chain_near = 0, chain_near32 = 0;
+
     // companion plan of J U^T J (J reverses the index order) for solves with U^T, built on the
     // first transposed solve and run on this plan's stream; tr_buf holds reversed b, reversed
     // x and the intermediate of the Cholesky solve (n*m each)
@@ -176,6 +183,7 @@ void block_free(sptrsv_plan_s* p);        // block.hip
 bool detect_grid(int n, const int* rowptr, const int* col, int64_t* dims);  // block.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
+void chain_free(sptrsv_plan_s* p);                                                        // chain.hip
 int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip
 int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s);  // solve.hip
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);

@@ -240,6 +240,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     supernodal_free(p);
     tile_free(p);
     block_free(p);
+    chain_free(p);
     void* bufs[] = {p->rowptr, p->col, p->val, p->val32, p->colptr, p->rowidx, p->cval, p->cval32,
                     p->indeg, p->levels, p->level_ptr, p->level_rows, p->lrowptr, p->lcol,
                     p->lval, p->lval32, p->ticket, p->indeg_work, p->left_sum, p->b32, p->x32,

@@ -122,12 +122,13 @@ def test_fp64_chain(plans, name):
     assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 4096 + 130, 9000])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 63, 64, 65, 2048 + 130, 4096 + 130, 9000])
 def test_chain_window_and_ring_edges(n):
-    """window boundaries, a partial last window, and entries older than the 4096-row ring"""
+    """window boundaries (32 rows), a partial last window, and entries on both sides of the
+    2048-row LDS ring's edge"""
     rng = np.random.default_rng(n)
     rows, cols, vals = [np.arange(n)], [np.arange(n)], [rng.uniform(2, 3, n)]
-    for off in (1, 2, 63, 64, 65, 4095, 4096, 4097, 5000):
+    for off in (1, 2, 31, 32, 33, 63, 64, 65, 2047, 2048, 2049, 4095, 4096, 4097, 5000):
         r = np.arange(max(n - off, 0))
         rows.append(r); cols.append(r + off); vals.append(rng.uniform(-0.2, 0.2, len(r)))
     rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
@@ -137,6 +138,33 @@ def test_chain_window_and_ring_edges(n):
         xt, b = gen.rhs_for(U, 1, seed=1)
         x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
         assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
+
+
+def test_chain_direct_path_windows():
+    """windows whose entries overflow a record (more than 448 entries within the ring, or more
+    than 64 older entries) are solved straight from the CSR, between windows that use records"""
+    n = 7000
+    rng = np.random.default_rng(5)
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [rng.uniform(2, 3, n)]
+    r = np.arange(n - 2049)
+    for off in (1, 2049):
+        rows.append(r); cols.append(r + off); vals.append(rng.uniform(-0.2, 0.2, len(r)))
+    band = np.arange(2000, 3000)  # 20 near entries per row: 640 per window
+    for off in range(2, 22):
+        rows.append(band); cols.append(band + off); vals.append(rng.uniform(-0.04, 0.04, len(band)))
+    far = np.arange(4000, 4500)  # 3 old entries per row: 96 per window
+    for off in (2100, 2200, 2300):
+        rows.append(far); cols.append(far + off); vals.append(rng.uniform(-0.1, 0.1, len(far)))
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    o = np.lexsort((cols, rows))
+    U = gen._from_rows(n, rows[o], cols[o], vals[o], "direct")
+    with capi.Plan.from_upper(U) as P:
+        xt, b = gen.rhs_for(U, 1, seed=3)
+        ref = oracle_solve(U, b)
+        x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
+        assert O.rel_error(ref, x) <= FP64_TOL
+        x2 = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))  # the records are reused
+        assert np.array_equal(x, x2)
 
 
 @pytest.mark.parametrize("group", [2, 8, 64])

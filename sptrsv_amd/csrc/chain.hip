@@ -6,19 +6,23 @@
 // dependent multiply-add and a cross-lane broadcast per row.  This solver shortens the chain
 // 32-fold with precomputed inverses:
 //   * rows are taken in windows of 32 from the last row up; window k holds rows hi-32 .. hi-1
-//     (hi = n - 32 k); lane l and lane l + 32 both own row hi - 1 - l (halves h = 0, 1);
+//     (hi = n - 32 k); lane L = l + 32 h owns row hi - 1 - l in both halves h = 0, 1;
 //   * the window's triangle T_k (its rows restricted to its own columns) is inverted once, when
-//     the layout is built (one wave per window, all windows in parallel), and kept as a packed
-//     lower-triangular 32 x 32 block A_k = T_k^-1 in window-lane order;
+//     the layout is built (one wave per window, all windows in parallel): A_k = T_k^-1;
 //   * per window the solve forms g = b - (products with every column outside the window, all
-//     solved before) and x = A_k g: a 32 x 32 product split between the two halves of the wave.
-// Every window has a fixed-size record, built once: A_k, its "near" entries (columns less than
-// 2048 rows above the window, whose x is still in an LDS ring; stored as ring offsets) and its
-// "old" entries (columns further up, whose x is read back from HBM).  One workgroup: wave 0 is
-// the only wave on the dependency chain and touches LDS only (x goes to the LDS ring); waves
-// 1..6 stage records through a three-step pipeline (record -> x of its old entries -> LDS slot,
-// 12 slots) and wave 7 copies finished windows from the ring to x in HBM, so that memory latency
-// stays off the chain.  Windows whose entries do not fit a record take a direct path.
+//     solved before) and x = A_k g, a 32 x 32 product split between the two halves of the wave.
+// Every window has a fixed-size record, built once, laid out lane-major so that each lane reads
+// its own part with conflict-free LDS loads: the lane's 16 entries of A_k, its "near" entries
+// (columns less than 2048 rows above the window, whose x is still in an LDS ring; stored as ring
+// indices) and its "old" entries (columns further up, whose x is read back from HBM).  A row's
+// outside entries alternate between its two lanes.
+// One workgroup: wave 0 is the only wave on the dependency chain.  It reads LDS only, holds
+// the next window's record in registers while it solves the current one, and sums across
+// halves with v_permlane32_swap, so a window costs two dependent LDS round trips (the ring
+// gather and the broadcast of g).  Waves 1..6 stage records three windows deep (record ->
+// x of its old entries -> LDS slot, 12 slots) and fold the old entries into b; wave 7 copies
+// finished windows from the ring to x in HBM.  Windows whose entries do not fit a record take a
+// direct path through the CSR.
 // x is formed in a different order than the serial solver's sums (A_k g instead of
 // substitution): it agrees with it to rounding (relative error <= 1e-12 in FP64).
 #include <hip/hip_runtime.h>
@@ -34,32 +38,43 @@ namespace sptrsv {
 namespace {
 
 constexpr int kW = 32;                    // rows per window
-constexpr int kPack = kW * (kW + 1) / 2;  // entries of a packed lower-triangular inverse
-constexpr int kRing = 2048;               // x values kept in LDS
-constexpr int kOld = 64;                  // old entries a record holds
+constexpr int kRing = 2048;               // x values kept in LDS; ring[kRing] is a constant 0
+constexpr int kNear = 4;                  // near entries per lane (8 per row); the layout assumes 4
+constexpr int kOldSlots = 2;              // old entries per lane (4 per row)
 constexpr int kSlots = 12;                // staged windows
 constexpr int kStagers = 6;               // stager waves
 constexpr int kChainThreads = 64 * (2 + kStagers);  // + the solving wave and the writer wave
-constexpr int kWriteChunk = 8;            // windows the writer copies to HBM at a time
-constexpr int kMaxNear = 448;             // near entries a record may hold (record <= 10 KiB)
+constexpr int kWriteChunk = 16;           // windows the writer copies to HBM at a time
 
-// packed position of A[l][j] (j <= l): column-major by j
-__host__ __device__ constexpr int wpos(int l, int j) { return j * kW - j * (j - 1) / 2 + (l - j); }
-
-// Byte layout of one window's record for value type T and near capacity E (multiple of 64).
+// Byte layout of one window's record for value type T; lane L = l + 32 h:
+//   A     T[8][64][2]      A_k[l][16 h + 2 i + t] at ((i * 64 + L) * 2 + t), 0 where the column
+//                          index exceeds l (A_k is lower triangular in lane order)
+//   nval  T[2][64][2]      near entry e of lane L at (((e >> 1) * 64 + L) * 2 + (e & 1)), 0 for
+//                          padding
+//   ncol  u16[64][4]       its ring index (column & (kRing - 1)) at (L * 4 + e); kRing for padding
+//   flags int[4]           [0] = 1: the window takes the direct path
+//   --- the part above is staged into an LDS slot ---
+//   oval  T[kOldSlots][64] old entries, folded into b by the stager
+//   ocol  int[kOldSlots][64]   their columns (-1: none)
 template <typename T>
 struct RecLayout {
-    int E;
-    __host__ __device__ int inv() const { return 0; }
-    __host__ __device__ int nval() const { return kPack * (int)sizeof(T); }
-    __host__ __device__ int oval() const { return nval() + E * (int)sizeof(T); }
-    __host__ __device__ int ocol() const { return oval() + kOld * (int)sizeof(T); }
-    __host__ __device__ int ncol() const { return ocol() + kOld * 4; }  // int16 ring offsets
-    __host__ __device__ int nrp() const { return ncol() + E * 2; }      // int16 x 33
-    __host__ __device__ int orp() const { return nrp() + 34 * 2; }      // int16 x 33
-    __host__ __device__ int flags() const { return orp() + 34 * 2; }    // int: 1 = direct path
-    __host__ __device__ int bytes() const { return (flags() + 4 + 15) & ~15; }
+    static constexpr int nval() { return 16 * 64 * (int)sizeof(T); }
+    static constexpr int ncol() { return nval() + kNear * 64 * (int)sizeof(T); }
+    static constexpr int flags() { return ncol() + 64 * kNear * 2; }
+    static constexpr int staged() { return flags() + 16; }
+    static constexpr int oval() { return staged(); }
+    static constexpr int ocol() { return oval() + kOldSlots * 64 * (int)sizeof(T); }
+    static constexpr int bytes() { return ocol() + kOldSlots * 64 * 4; }
+    static constexpr int pieces() { return (staged() / 16 + 63) / 64; }  // 16-byte pieces per stager lane
 };
+static_assert(RecLayout<double>::staged() % 16 == 0 && RecLayout<double>::bytes() % 16 == 0, "record alignment");
+static_assert(RecLayout<float>::staged() % 16 == 0 && RecLayout<float>::bytes() % 16 == 0, "record alignment");
+static_assert(kNear == 4, "ncol holds four ring indices per lane");
+
+// position of A_k[l][j] in the record, in elements of T
+__host__ __device__ constexpr int apos(int l, int j) {
+    return ((((j & 15) >> 1) * 64 + l + 32 * (j >> 4)) * 2) + (j & 1);
+}
 
 // ---- layout: the inverse of every window's triangle ----------------------------------------
 // One wave per window: lane l < 32 scatters row l's in-window entries into T (LDS), then lane j
@@ -102,14 +117,12 @@ __global__ __launch_bounds__(64) void k_chain_invert(int n, int nwin, const int*
         }
     }
     __syncthreads();
+    // lane L writes A[l][16 h + jj], jj = 0..15 (zeros above the diagonal)
     TA* dst = reinterpret_cast<TA*>(rec + (size_t)k * rec_bytes);
-    for (int e = lane; e < kPack; e += 64) {
-        int j = 0, base = 0;  // invert wpos: column j holds kW - j entries
-        while (base + (kW - j) <= e) {
-            base += kW - j;
-            ++j;
-        }
-        dst[e] = (TA)A[j + (e - base)][j];
+    const int l = lane & (kW - 1), h = lane >> 5;
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * h + jj;
+        dst[apos(l, j)] = (TA)(j <= l ? A[l][j] : 0.0);
     }
 }
 
@@ -117,9 +130,7 @@ template <typename T>
 struct ChainArgs {
     int n;
     int nwin;
-    int rec_bytes;
-    RecLayout<T> L;
-    const char* rec;    // nwin records
+    const char* rec;    // nwin records of RecLayout<T>::bytes()
     const int* rowptr;  // direct path
     const int* col;
     const T* val;
@@ -128,29 +139,53 @@ struct ChainArgs {
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // a native 16-byte register piece
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// (value of lane L & 31) + (value of lane (L & 31) + 32), the same in every lane: the two halves
+// are exchanged with v_permlane32_swap (no LDS round trip)
+__device__ __forceinline__ double half_sum(double v) {
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double(ph[0], pl[0]) + __hiloint2double(ph[1], pl[1]);
+}
+__device__ __forceinline__ float half_sum(float v) {
+    const unsigned u = __float_as_uint(v);
+    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
 
 // What a stager holds for one window between its pipeline steps.
 template <typename T, int NP>
 struct Staged {
-    u32x4 piece[NP];  // the record, 16-byte pieces, lane-strided
-    T bv;             // b of row hi - 1 - lane (lanes < 32)
-    int ocol;         // column of old entry `lane` (-1: none)
-    T oldx;           // x of that column
+    u32x4 piece[NP];    // the staged part of the record, 16-byte pieces, lane-strided
+    T bv;               // b of row hi - 1 - (lane & 31)
+    int ocol[kOldSlots];
+    T oval[kOldSlots];
+    T oldx[kOldSlots];  // x of those columns
 };
 
-// Step 1: the record, b and the old columns.  Loads are unconditional (indices clamped into
+// Step 1: the record, b and the old entries.  Loads are unconditional (indices clamped into
 // range) so that the registers are not spilled and each wait falls at the first use.
 template <typename T, int NP>
 __device__ __forceinline__ void stage_load(const ChainArgs<T>& a, int k, int lane, Staged<T, NP>& s) {
+    using L = RecLayout<T>;
     const int kk = min(k, a.nwin - 1);
-    const char* r = a.rec + (size_t)kk * a.rec_bytes;
+    const char* r = a.rec + (size_t)kk * L::bytes();
     const u32x4* src = reinterpret_cast<const u32x4*>(r);
-    const int npieces = a.rec_bytes / 16;
+    constexpr int npieces = L::staged() / 16;
 #pragma unroll
     for (int q = 0; q < NP; ++q) s.piece[q] = src[min(lane + 64 * q, npieces - 1)];
     const int row = max(0, a.n - kW * kk - 1 - (lane & (kW - 1)));
     s.bv = a.b[row];
-    s.ocol = reinterpret_cast<const int*>(r + a.L.ocol())[lane];
+#pragma unroll
+    for (int q = 0; q < kOldSlots; ++q) {
+        s.ocol[q] = reinterpret_cast<const int*>(r + L::ocol())[q * 64 + lane];
+        s.oval[q] = reinterpret_cast<const T*>(r + L::oval())[q * 64 + lane];
+    }
 }
 
 // Step 2: x of the old entries.  They were solved at least 2048 rows (64 windows) before the
@@ -159,46 +194,94 @@ template <typename T, int NP>
 __device__ __forceinline__ void stage_old(const ChainArgs<T>& a, int k, Staged<T, NP>& s, int* published) {
     while (__hip_atomic_load(published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < min(k, a.nwin) - 63)
         __builtin_amdgcn_s_sleep(2);
-    s.oldx = from_bits<T>(poll_bits(s.ocol >= 0 ? a.x + s.ocol : a.b));  // unused entries read b[0]
+#pragma unroll
+    for (int q = 0; q < kOldSlots; ++q) {
+        const T v = from_bits<T>(poll_bits(s.ocol[q] >= 0 ? a.x + s.ocol[q] : a.b));  // unused slots read b[0]
+        s.oldx[q] = s.ocol[q] >= 0 ? v : T(0);
+    }
 }
 
-// Step 3: into the window's LDS slot once the slot's previous window is solved.
+// Step 3: into the window's LDS slot once the slot's previous window is solved; b minus the old
+// entries' products goes to slot_s (row l's b in lane l, only the products in lane l + 32).
 template <typename T, int NP>
 __device__ __forceinline__ void stage_deposit(const ChainArgs<T>& a, int k, int lane, const Staged<T, NP>& s,
-                                              char* slot, T* slot_b, T* slot_oldx, int* slot_full, int* done) {
+                                              char* slot, T* slot_s, int* slot_full, int* done) {
     if (k >= a.nwin) return;
     while (__hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < k - kSlots + 1)
         __builtin_amdgcn_s_sleep(1);
     u32x4* dst = reinterpret_cast<u32x4*>(slot);
-    const int npieces = a.rec_bytes / 16;
+    constexpr int npieces = RecLayout<T>::staged() / 16;
 #pragma unroll
     for (int q = 0; q < NP; ++q)
         if (lane + 64 * q < npieces) dst[lane + 64 * q] = s.piece[q];
-    if (lane < kW) slot_b[lane] = s.bv;
-    slot_oldx[lane] = s.oldx;
+    const bool valid = a.n - kW * k - 1 - (lane & (kW - 1)) >= 0;
+    T acc = (lane < kW && valid) ? s.bv : T(0);
+#pragma unroll
+    for (int q = 0; q < kOldSlots; ++q) acc = fma(-s.oval[q], s.oldx[q], acc);
+    slot_s[lane] = acc;
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) __hip_atomic_store(&slot_full[k % kSlots], k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The solving wave's registers for one window.
+template <typename T>
+struct Win {
+    T A[16];      // A_k[l][16 h + jj]
+    T nv[kNear];  // near entries
+    int nc[kNear];
+    T s;          // b minus the old entries' products (lane share)
+    int flags;
+};
+
+template <typename T>
+__device__ __forceinline__ void win_load(const char* slot, const T* slot_s, int lane, Win<T>& w) {
+    using L = RecLayout<T>;
+    using P = typename Pair<T>::type;
+    const P* a2 = reinterpret_cast<const P*>(slot);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const P v = a2[i * 64 + lane];
+        w.A[2 * i] = v.x;
+        w.A[2 * i + 1] = v.y;
+    }
+    // 13 LDS reads in all: issued behind the 8 reads of g, they stay within the 15 that the
+    // LDS counter can leave outstanding, so the product with g waits for none of them
+    const P* nv = reinterpret_cast<const P*>(slot + L::nval());
+#pragma unroll
+    for (int i = 0; i < kNear / 2; ++i) {
+        const P v = nv[i * 64 + lane];
+        w.nv[2 * i] = v.x;
+        w.nv[2 * i + 1] = v.y;
+    }
+    const u16x4 c = reinterpret_cast<const u16x4*>(slot + L::ncol())[lane];
+    w.nc[0] = c.x;
+    w.nc[1] = c.y;
+    w.nc[2] = c.z;
+    w.nc[3] = c.w;
+    w.s = slot_s[lane];
+    w.flags = *reinterpret_cast<const int*>(slot + L::flags());
+}
+
 template <typename T, int NP>
 __global__ __launch_bounds__(kChainThreads, 1) void k_chain(ChainArgs<T> a) {
-#pragma clang fp contract(off)
+    using P = typename Pair<T>::type;
     constexpr int kSlotBytes = NP * 64 * 16;
-    __shared__ T ring[kRing];
+    __shared__ T ring[kRing + 1];
     __shared__ __align__(16) char slot[kSlots][kSlotBytes];
-    __shared__ T slot_b[kSlots][kW];
-    __shared__ T slot_oldx[kSlots][kOld];
-    __shared__ T gbuf[2][kW];
+    __shared__ T slot_s[kSlots][64];
+    __shared__ __align__(16) T gbuf[2][kW];
     __shared__ int slot_full[kSlots];  // window + 1 staged in the slot
     __shared__ int done;               // windows solved
     __shared__ int written;            // windows copied from the ring to HBM (stores issued)
     __shared__ int published;          // windows whose x stores have completed
     // the wave's role as a scalar: the branches below are then uniform, and the compiler keeps
-    // the roles' code apart instead of linearising it (which would carry the stagers' pending
-    // loads into the solving wave's waits)
+    // the roles' code apart instead of linearising it
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid < kSlots) slot_full[tid] = 0;
-    if (tid == 0) done = written = published = 0;
+    if (tid == 0) {
+        done = written = published = 0;
+        ring[kRing] = T(0);  // what padding entries read
+    }
     __syncthreads();
 
     if (wave == kStagers + 1) {
@@ -229,8 +312,7 @@ __global__ __launch_bounds__(kChainThreads, 1) void k_chain(ChainArgs<T> a) {
         for (int k = s0; k < a.nwin; k += kStagers) {
             stage_load(a, k + 2 * kStagers, lane, p2);
             stage_old(a, k + kStagers, p1, &published);
-            stage_deposit(a, k, lane, p0, slot[k % kSlots], slot_b[k % kSlots], slot_oldx[k % kSlots], slot_full,
-                          &done);
+            stage_deposit(a, k, lane, p0, slot[k % kSlots], slot_s[k % kSlots], slot_full, &done);
             p0 = p1;
             p1 = p2;
         }
@@ -239,65 +321,91 @@ __global__ __launch_bounds__(kChainThreads, 1) void k_chain(ChainArgs<T> a) {
 
     // ---- wave 0: the windows in order -------------------------------------------------------
     const int l = lane & (kW - 1), h = lane >> 5;
+    while (__hip_atomic_load(&slot_full[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 1)
+        __builtin_amdgcn_s_sleep(1);
+    Win<T> cur, nxt;
+    win_load(slot[0], slot_s[0], lane, cur);
+    int wr = 0;  // last value of `written` seen
     int hi = a.n;
     for (int k = 0; k < a.nwin; ++k) {
         const int r = hi - 1 - l;
         const bool valid = r >= 0;
-        const int sl = k % kSlots;
-        while (__hip_atomic_load(&slot_full[sl], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != k + 1)
-            __builtin_amdgcn_s_sleep(1);
-        const char* rec = slot[sl];
-        T acc = T(0);
-        if (*reinterpret_cast<const int*>(rec + a.L.flags()) == 0) {
-            const T* nval = reinterpret_cast<const T*>(rec + a.L.nval());
-            const short* ncol = reinterpret_cast<const short*>(rec + a.L.ncol());
-            const short* nrp = reinterpret_cast<const short*>(rec + a.L.nrp());
-            const T* oval = reinterpret_cast<const T*>(rec + a.L.oval());
-            const short* orp = reinterpret_cast<const short*>(rec + a.L.orp());
-            const T* oldx = slot_oldx[sl];
-            for (int e = nrp[l] + h; e < nrp[l + 1]; e += 2) acc += nval[e] * ring[(hi + ncol[e]) & (kRing - 1)];
-            for (int e = orp[l] + h; e < orp[l + 1]; e += 2) acc += oval[e] * oldx[e];
+        // whether the next window is staged: read now, looked at after the gather below (its
+        // wait covers this read too)
+        const int sl = (k + 1) % kSlots;
+        int staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // g: this lane's share of b minus the products with columns outside the window
+        T acc = cur.s;
+        if (cur.flags == 0) {
+            T part = T(0);
+#pragma unroll
+            for (int e = 0; e < kNear; ++e) part = fma(cur.nv[e], ring[cur.nc[e]], part);
+            acc -= part;
         } else if (valid) {
-            // entries beyond the record's capacity: straight from the CSR; x of old columns (at
-            // least 64 windows back) is read from HBM once the writer has published it
+            // entries beyond the record's capacity: straight from the CSR, alternating between
+            // the row's two lanes; x of old columns (at least 64 windows back) is read from HBM
+            // once the writer has published it
             while (__hip_atomic_load(&published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < k - 63)
                 __builtin_amdgcn_s_sleep(2);
+            T part = T(0);
             for (int p = a.rowptr[r] + 1 + h; p < a.rowptr[r + 1]; p += 2) {
                 const int c = a.col[p];
                 if (c < hi) continue;
                 const T xv = c < hi + kRing ? ring[c & (kRing - 1)] : from_bits<T>(poll_bits(a.x + c));
-                acc += a.val[p] * xv;
+                part = fma(a.val[p], xv, part);
+            }
+            acc -= part;
+        }
+        const T g = half_sum(acc);
+        if (k + 1 < a.nwin) {
+            while (staged != k + 2) {
+                __builtin_amdgcn_s_sleep(1);
+                staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        const T g = slot_b[sl][l] - (acc + __shfl_xor(acc, 32));
+        // The slot's contents were stored before its flag was (the stager's release), and this
+        // wave's LDS reads execute in issue order: the reads below see them.  The empty asm keeps
+        // the compiler from moving them above the flag test.
+        __asm__ volatile("" ::: "memory");
         if (h == 0) gbuf[k & 1][l] = g;
         __builtin_amdgcn_wave_barrier();
-        const T* A = reinterpret_cast<const T*>(rec + a.L.inv());
-        const T* gb = gbuf[k & 1];
-        T s = T(0);
+        const P* gb = reinterpret_cast<const P*>(gbuf[k & 1] + 16 * h);
+        P gv[8];
 #pragma unroll
-        for (int jj = 0; jj < kW / 2; ++jj) {
-            const int j = (kW / 2) * h + jj;
-            const T av = A[j <= l ? wpos(l, j) : wpos(j, j)];
-            s += (j <= l ? av : T(0)) * gb[j];
+        for (int i = 0; i < 8; ++i) gv[i] = gb[i];
+        // the next window's record, read while the broadcast of g is in flight (after the last
+        // window: a read of a slot that is not used)
+        win_load(slot[sl], slot_s[sl], lane, nxt);
+        // x = A_k g: 16 columns per half, four partial sums
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            s0 = fma(cur.A[2 * i], gv[i].x, s0);
+            s1 = fma(cur.A[2 * i + 1], gv[i].y, s1);
+            s2 = fma(cur.A[2 * i + 2], gv[i + 1].x, s2);
+            s3 = fma(cur.A[2 * i + 3], gv[i + 1].y, s3);
         }
-        const T xv = s + __shfl_xor(s, 32);
+        const T xv = half_sum((s0 + s1) + (s2 + s3));
         // the ring entries about to be replaced (window k - 64) must have been copied to HBM
-        while (__hip_atomic_load(&written, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < k - 63)
-            __builtin_amdgcn_s_sleep(1);
+        if (wr < k - 63) {
+            while ((wr = __hip_atomic_load(&written, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < k - 63)
+                __builtin_amdgcn_s_sleep(1);
+        }
         if (h == 0 && valid) ring[r & (kRing - 1)] = xv;
-        // release: the slot's LDS reads have returned and x is in the ring, so the stager may
-        // refill the slot and the writer may copy the window (this wave has no HBM accesses in
-        // flight here, so the release waits on LDS only)
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) __hip_atomic_store(&done, k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // This wave's LDS accesses complete in issue order, so the ring store and every read of
+        // the window's slot precede the store below: a relaxed store suffices, and the empty asm
+        // keeps the compiler from moving memory accesses across it.
+        __asm__ volatile("" ::: "memory");
+        if (lane == 0) __hip_atomic_store(&done, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        cur = nxt;
         hi -= kW;
     }
 }
 
-// Host side: every window's near and old entries, the record capacity, and the records.
+// Host side: every window's near and old entries and the records.
 template <typename T>
-int build_records(sptrsv_plan_s* P, char** out, int* rec_bytes, int* near_cap) {
+int build_records(sptrsv_plan_s* P, char** out) {
+    using L = RecLayout<T>;
     const int n = P->n, nwin = (n + kW - 1) / kW;
     std::vector<int> rp((size_t)n + 1), col((size_t)P->nnz);
     std::vector<double> val((size_t)P->nnz);
@@ -305,56 +413,59 @@ int build_records(sptrsv_plan_s* P, char** out, int* rec_bytes, int* near_cap) {
     SPT_HIP(hipMemcpy(rp.data(), P->rowptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
     SPT_HIP(hipMemcpy(col.data(), P->col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
     SPT_HIP(hipMemcpy(val.data(), P->val, val.size() * sizeof(double), hipMemcpyDeviceToHost));
-    // near capacity: the largest window's near count, in steps of 64, at most kMaxNear
-    int emax = 0;
-    for (int k = 0; k < nwin; ++k) {
-        const int hi = n - kW * k, lo = std::max(0, hi - kW);
-        int cnt = 0;
-        for (int r = lo; r < hi; ++r)
-            for (int p = rp[r] + 1; p < rp[r + 1]; ++p) cnt += (col[p] >= hi && col[p] < hi + kRing) ? 1 : 0;
-        emax = std::max(emax, cnt);
-    }
-    RecLayout<T> L{std::min(kMaxNear, std::max(64, (emax + 63) & ~63))};
-    const int rb = L.bytes();
+    const int rb = L::bytes();
     std::vector<char> host((size_t)nwin * rb, 0);
     for (int k = 0; k < nwin; ++k) {
         char* R = host.data() + (size_t)k * rb;
         const int hi = n - kW * k, lo = std::max(0, hi - kW);
-        T* nv = reinterpret_cast<T*>(R + L.nval());
-        T* ov = reinterpret_cast<T*>(R + L.oval());
-        int* oc = reinterpret_cast<int*>(R + L.ocol());
-        short* nc = reinterpret_cast<short*>(R + L.ncol());
-        short* nrp = reinterpret_cast<short*>(R + L.nrp());
-        short* orp = reinterpret_cast<short*>(R + L.orp());
-        int* flags = reinterpret_cast<int*>(R + L.flags());
-        for (int q = 0; q < kOld; ++q) oc[q] = -1;
-        int ne = 0, no = 0;
-        for (int l = 0; l < kW; ++l) {
-            nrp[l] = (short)std::min(ne, L.E);
-            orp[l] = (short)std::min(no, kOld);
+        T* nv = reinterpret_cast<T*>(R + L::nval());
+        unsigned short* nc = reinterpret_cast<unsigned short*>(R + L::ncol());
+        int* flags = reinterpret_cast<int*>(R + L::flags());
+        T* ov = reinterpret_cast<T*>(R + L::oval());
+        int* oc = reinterpret_cast<int*>(R + L::ocol());
+        for (int q = 0; q < kNear * 64; ++q) nc[q] = (unsigned short)kRing;
+        for (int q = 0; q < kOldSlots * 64; ++q) oc[q] = -1;
+        auto npos = [](int e, int lane) { return (((e >> 1) * 64 + lane) * 2) + (e & 1); };
+        bool direct = false;
+        for (int l = 0; l < kW && !direct; ++l) {
             const int r = hi - 1 - l;
-            if (r < lo) continue;
+            if (r < lo) break;
+            int ne = 0, no = 0;  // entry i of a kind goes to lane l + 32 (i & 1), slot i >> 1
             for (int p = rp[r] + 1; p < rp[r + 1]; ++p) {
                 const int c = col[p];
                 if (c < hi) continue;
                 if (c < hi + kRing) {
-                    if (ne < L.E) {
-                        nv[ne] = (T)val[(size_t)p];
-                        nc[ne] = (short)(c - hi);
+                    const int e = ne >> 1, L2 = l + 32 * (ne & 1);
+                    if (e >= kNear) {
+                        direct = true;
+                        break;
                     }
+                    nv[npos(e, L2)] = (T)val[(size_t)p];
+                    nc[L2 * 4 + e] = (unsigned short)(c & (kRing - 1));
                     ++ne;
                 } else {
-                    if (no < kOld) {
-                        ov[no] = (T)val[(size_t)p];
-                        oc[no] = c;
+                    const int q = no >> 1, L2 = l + 32 * (no & 1);
+                    if (q >= kOldSlots) {
+                        direct = true;
+                        break;
                     }
+                    ov[q * 64 + L2] = (T)val[(size_t)p];
+                    oc[q * 64 + L2] = c;
                     ++no;
                 }
             }
         }
-        nrp[kW] = (short)std::min(ne, L.E);
-        orp[kW] = (short)std::min(no, kOld);
-        *flags = (ne > L.E || no > kOld) ? 1 : 0;
+        if (direct) {  // the solving wave reads the CSR for this window: no near or old entries
+            for (int q = 0; q < kNear * 64; ++q) {
+                nv[q] = T(0);
+                nc[q] = (unsigned short)kRing;
+            }
+            for (int q = 0; q < kOldSlots * 64; ++q) {
+                ov[q] = T(0);
+                oc[q] = -1;
+            }
+        }
+        *flags = direct ? 1 : 0;
     }
     char* d = nullptr;
     SPT_HIP(hipMalloc((void**)&d, host.size()));
@@ -367,30 +478,17 @@ int build_records(sptrsv_plan_s* P, char** out, int* rec_bytes, int* near_cap) {
         return fail(SPTRSV_ERR_HIP, std::string("chain layout: ") + hipGetErrorString(e));
     }
     *out = d;
-    *rec_bytes = rb;
-    *near_cap = L.E;
     return SPTRSV_OK;
 }
 
-template <typename T, int NP>
-int launch_chain(sptrsv_plan_s* P, const char* rec, int rec_bytes, int near_cap, const T* val, const T* b, T* x) {
+template <typename T>
+int run_chain(sptrsv_plan_s* P, const char* rec, const T* val, const T* b, T* x) {
+    if (P->n == 0) return SPTRSV_OK;  // no windows: nothing to stage
     const int nwin = (P->n + kW - 1) / kW;
-    ChainArgs<T> a{P->n, nwin, rec_bytes, RecLayout<T>{near_cap}, rec, P->rowptr, P->col, val, b, x};
-    k_chain<T, NP><<<1, kChainThreads, 0, P->stream>>>(a);
+    ChainArgs<T> a{P->n, nwin, rec, P->rowptr, P->col, val, b, x};
+    k_chain<T, RecLayout<T>::pieces()><<<1, kChainThreads, 0, P->stream>>>(a);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
-}
-
-// The kernel instance is picked by the record size: NP 16-byte pieces per stager lane.
-template <typename T>
-int run_chain(sptrsv_plan_s* P, const char* rec, int rec_bytes, int near_cap, const T* val, const T* b, T* x) {
-    if (P->n == 0) return SPTRSV_OK;  // no windows: nothing to stage
-    const int np = (rec_bytes / 16 + 63) / 64;
-    if (np > 10) return fail(SPTRSV_ERR_UNSUPPORTED, "chain layout: record larger than an LDS slot");
-    if (np <= 4) return launch_chain<T, 4>(P, rec, rec_bytes, near_cap, val, b, x);
-    if (np <= 6) return launch_chain<T, 6>(P, rec, rec_bytes, near_cap, val, b, x);
-    if (np <= 8) return launch_chain<T, 8>(P, rec, rec_bytes, near_cap, val, b, x);
-    return launch_chain<T, 10>(P, rec, rec_bytes, near_cap, val, b, x);
 }
 
 }  // namespace
@@ -404,17 +502,17 @@ void chain_free(sptrsv_plan_s* P) {
 int chain_solve(sptrsv_plan_s* P, int precision, const double* d_b, double* d_x, int m) {
     if (m != 1) return fail(SPTRSV_ERR_UNSUPPORTED, "chain variant solves one right-hand side");
     if (precision != SPTRSV_FP64) return fail(SPTRSV_ERR_UNSUPPORTED, "chain variant is FP64 through sptrsv_solve");
-    if (!P->chain_rec) SPT_CHECK(build_records<double>(P, &P->chain_rec, &P->chain_rec_bytes, &P->chain_near));
+    if (!P->chain_rec) SPT_CHECK(build_records<double>(P, &P->chain_rec));
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    SPT_CHECK(run_chain<double>(P, P->chain_rec, P->chain_rec_bytes, P->chain_near, P->val, d_b, d_x));
+    SPT_CHECK(run_chain<double>(P, P->chain_rec, P->val, d_b, d_x));
     end_timed(P, stop);
     return SPTRSV_OK;
 }
 
 int chain_solve32(sptrsv_plan_s* P, const float* d_b, float* d_x) {
-    if (!P->chain_rec32) SPT_CHECK(build_records<float>(P, &P->chain_rec32, &P->chain_rec32_bytes, &P->chain_near32));
-    return run_chain<float>(P, P->chain_rec32, P->chain_rec32_bytes, P->chain_near32, P->val32, d_b, d_x);
+    if (!P->chain_rec32) SPT_CHECK(build_records<float>(P, &P->chain_rec32));
+    return run_chain<float>(P, P->chain_rec32, P->val32, d_b, d_x);
 }
 
 }  // namespace sptrsv

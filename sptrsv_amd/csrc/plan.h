@@ -126,8 +126,6 @@ struct sptrsv_plan_s {
     // entries), built on first use; FP64 and FP32 values
     char* chain_rec = nullptr;
     char* chain_rec32 = nullptr;
-    int chain_rec_bytes = 0, chain_rec32_bytes = 0;
-    int chain_near = 0, chain_near32 = 0;
 
     // companion plan of J U^T J (J reverses the index order) for solves with U^T, built on the
     // first transposed solve and run on this plan's stream; tr_buf holds reversed b, reversed

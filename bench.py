@@ -180,6 +180,17 @@ def cpu_baseline(workloads, headline, m):
     }
 
 
+def plan_timing(O, U, P, plan_s):
+    """Plan creation (upload + device analysis) and its parts, next to the oracle's level
+    analysis on the host (setLevelUpAndGetMaxLevel, src/core.cpp:244-270; one core)."""
+    a_ms, l_ms = P.analysis_time()
+    t0 = time.perf_counter()
+    O.levels(U.rowptr, U.col)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    return {"plan_ms": round(plan_s * 1e3, 2), "analysis_ms": round(a_ms, 2), "levels_ms": round(l_ms, 2),
+            "levels": P.n_levels, "oracle_host_levels_ms": round(host_ms, 2)}
+
+
 def bench_mixed(capi, gen, names, out):
     """Wall time of the refined solve (reduced-precision solves + FP64 residual and update on the
     GPU, stopped at an estimated relative error <= 1e-13) against the plain FP64 solve of the
@@ -445,7 +456,10 @@ def main():
     variant = names[args.variant]
     U, desc = make_workload(args.workload)
     m = args.m
+    t0 = time.perf_counter()
     P = capi.Plan.from_upper(U)
+    analysis = {args.workload: plan_timing(O, U, P, time.perf_counter() - t0)}
+    progress(f"analysis {args.workload}: {analysis[args.workload]}")
     if variant == capi.VAR_SUPERNODAL:
         P.build_supernodes(U.snodes)
     xt, b = gen.rhs_for(U, m, seed=1, layout="row")
@@ -479,7 +493,10 @@ def main():
             try:
                 if wl not in cache:
                     UU, _ = make_workload(wl)
+                    t0 = time.perf_counter()
                     cache[wl] = (UU, capi.Plan.from_upper(UU))
+                    analysis[wl] = plan_timing(O, UU, cache[wl][1], time.perf_counter() - t0)
+                    progress(f"analysis {wl}: {analysis[wl]}")
                     if wl.startswith("supernodal"):
                         cache[wl][1].build_supernodes(UU.snodes)
                 UU, PP = cache[wl]
@@ -618,6 +635,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "extras": extras,
+        "analysis": analysis,
         "mixed_refine": mixed,
         "kappa_sweep": sweep,
         "batched": batch,

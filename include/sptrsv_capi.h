@@ -128,6 +128,11 @@ int sptrsv_plan_create_csc(int n, int64_t nnz, const int* colptr, const int* row
                            const double* val, sptrsv_plan_t* plan);
 int sptrsv_plan_destroy(sptrsv_plan_t plan);
 int sptrsv_plan_info(sptrsv_plan_t plan, int* n, int64_t* nnz, int* n_levels);
+/* Host wall time of the plan's analysis, in ms (no reference counterpart; the reference counts
+ * its analysis inside barrier_upper, src/core.cpp:375): transpose, in-degree, levels, level sets
+ * and level-ordered CSR, i.e. everything sptrsv_plan_create_csr does after the upload.
+ * levels_ms: the level computation alone. */
+int sptrsv_plan_analysis_time(sptrsv_plan_t plan, double* analysis_ms, double* levels_ms);
 
 /* Supernodal layout.  snodes[0..sn] are row boundaries (reference .snodes semantics,
  * src/utils.cpp:84-111); every supernode's rows must hold a dense triangle followed by one

@@ -108,6 +108,7 @@ def lib():
         "sptrsv_plan_create_csc": (_i, [_i, _i64, _vp, _vp, _vp, ctypes.POINTER(_PLAN)]),
         "sptrsv_plan_destroy": (_i, [_PLAN]),
         "sptrsv_plan_info": (_i, [_PLAN, ctypes.POINTER(_i), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
+        "sptrsv_plan_analysis_time": (_i, [_PLAN, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "sptrsv_plan_get_indegree": (_i, [_PLAN, _vp]),
         "sptrsv_plan_get_levels": (_i, [_PLAN, _vp]),
         "sptrsv_plan_get_blocks": (_i, [_PLAN, _vp, _vp, _vp, _vp]),
@@ -317,6 +318,12 @@ class Plan:
         n, nnz, L = _i(0), _i64(0), _i(0)
         check(lib().sptrsv_plan_info(self.handle, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(L)), "info")
         self.n, self.nnz, self.n_levels = n.value, nnz.value, L.value
+
+    def analysis_time(self):
+        """(analysis ms, level computation ms): host wall time of the plan's device analysis."""
+        a, l = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(lib().sptrsv_plan_analysis_time(self.handle, ctypes.byref(a), ctypes.byref(l)), "analysis_time")
+        return a.value, l.value
 
     @classmethod
     def from_csr(cls, rowptr, col, val):

@@ -8,6 +8,8 @@
 // All integer results are exact; the tests compare them bit for bit with the oracle.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <hipcub/hipcub.hpp>
 #include <string>
 #include <vector>
@@ -54,8 +56,14 @@ __global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict
 }
 
 // Level of every row by a sync-free max-plus sweep: rows are taken in descending order by
-// wave tickets; a row's level is 1 + the deepest level among its off-diagonal columns (1
-// for a diagonal-only row), exactly the reference's rule.  level 0 marks "not yet known".
+// wave tickets; a row's level is 1 + the deepest level among its off-diagonal columns (1 for a
+// diagonal-only row), exactly the reference's rule.  level 0 marks "not yet known".  Each lane
+// walks its row from the far end (the largest columns belong to rows earlier in the sweep, which
+// are known first), kLevelWin entries at a time: their columns are read once per window and
+// their levels are polled together, so a row waits one device round trip after its last
+// dependency is known, not one per entry.
+constexpr int kLevelWin = 8;
+
 __global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ rowptr,
                                                 const int* __restrict__ col, int* levels,
                                                 int* ctrl, uint64_t limit) {
@@ -63,31 +71,52 @@ __global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ r
     const int t = wave_ticket(&ctrl[0]);
     const int row = n - 1 - (t * kWave + lane);
     bool done = row < 0;
-    int p = 0, end = 0, deepest = 1, mine = 0;
+    int first = 0, p = 0, deepest = 0, mine = 0;
+    bool has = false;
     if (!done) {
-        p = rowptr[row] + 1;
-        end = rowptr[row + 1];
+        first = rowptr[row] + 1;
+        p = rowptr[row + 1];
+        has = p > first;
     }
+    int c[kLevelWin];
+    bool need_cols = true;
     Watchdog wd;
     wd.arm(&ctrl[1], limit);
     int idle = 0;
     while (true) {
         bool progressed = false;
-        if (!done) {
-            while (p < end) {
-                const int lv = __hip_atomic_load(&levels[col[p]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (lv == 0) break;
-                deepest = lv > deepest ? lv : deepest;
-                ++p;
+        if (!done && p > first) {
+            // window [q0, p): entry j is p - 1 - j; positions below q0 repeat q0 (harmless)
+            const int q0 = max(first, p - kLevelWin);
+            if (need_cols) {
+#pragma unroll
+                for (int j = 0; j < kLevelWin; ++j) c[j] = col[max(q0, p - 1 - j)];
+                need_cols = false;
+            }
+            int lv[kLevelWin];
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j)
+                lv[j] = __hip_atomic_load(&levels[c[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool all = true;
+            int mx = deepest;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) {
+                all = all && lv[j] != 0;
+                mx = lv[j] > mx ? lv[j] : mx;
+            }
+            if (all) {
+                deepest = mx;
+                p = q0;
+                need_cols = true;
                 progressed = true;
             }
-            if (p >= end) {
-                const int lev = (end - (rowptr[row] + 1) > 0) ? 1 + deepest : 1;
-                __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                mine = lev;
-                done = true;
-                progressed = true;
-            }
+        }
+        if (!done && p <= first) {
+            const int lev = has ? 1 + deepest : 1;
+            __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            mine = lev;
+            done = true;
+            progressed = true;
         }
         if (__all(done)) break;
         if (!__any(progressed)) {
@@ -211,12 +240,22 @@ int analyse(sptrsv_plan_s* p) {
     // ---- levels --------------------------------------------------------------------------
     k_fill_int<<<grid_for(n), 256, 0, s>>>(p->levels, 0, n);
     k_fill_int<<<1, 64, 0, s>>>(p->ticket, 0, 64);
-    const int waves = (n + kWave - 1) / kWave;
-    const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
-    k_levels<<<(waves + 3) / 4, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
+    // The sync-free sweep costs about one device round trip (~2 us) per level, the windowed
+    // sweep of one wave about 0.3 us per 32 rows.  depth_lb bounds the level count from below:
+    // the windowed sweep is taken once that bound makes it the cheaper one.
+    SPT_HIP(hipStreamSynchronize(s));
+    const auto tl = std::chrono::steady_clock::now();
+    if (p->depth_lb >= 2048 && (int64_t)n <= 256 * (int64_t)p->depth_lb) {
+        SPT_CHECK(chain_levels(p, p->ticket));
+    } else {
+        const int waves = (n + kWave - 1) / kWave;
+        const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
+        k_levels<<<(waves + 3) / 4, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
+    }
     int ctrl[3] = {0, 0, 0};
     SPT_HIP(hipMemcpyAsync(ctrl, p->ticket, sizeof(ctrl), hipMemcpyDeviceToHost, s));
     SPT_HIP(hipStreamSynchronize(s));
+    p->levels_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
     if (ctrl[1] != 0)
         return fail(SPTRSV_ERR_DEADLOCK, "level analysis watchdog fired");
     const int L = ctrl[2];

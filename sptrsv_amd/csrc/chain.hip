@@ -20,7 +20,7 @@
 // the next window's record in registers while it solves the current one, and sums across
 // halves with v_permlane32_swap, so a window costs two dependent LDS round trips (the ring
 // gather and the broadcast of g).  Waves 1..6 stage records three windows deep (record ->
-// x of its old entries -> LDS slot, 12 slots) and fold the old entries into b; wave 7 copies
+// x of its old entries -> LDS slot, 10 slots) and fold the old entries into b; wave 7 copies
 // finished windows from the ring to x in HBM.  Windows whose entries do not fit a record take a
 // direct path through the CSR.
 // x is formed in a different order than the serial solver's sums (A_k g instead of
@@ -39,9 +39,9 @@ namespace {
 
 constexpr int kW = 32;                    // rows per window
 constexpr int kRing = 2048;               // x values kept in LDS; ring[kRing] is a constant 0
-constexpr int kNear = 4;                  // near entries per lane (8 per row); the layout assumes 4
+constexpr int kNear = 6;                  // near entries per lane (12 per row); the layout assumes 6
 constexpr int kOldSlots = 2;              // old entries per lane (4 per row)
-constexpr int kSlots = 12;                // staged windows
+constexpr int kSlots = 10;                // staged windows
 constexpr int kStagers = 6;               // stager waves
 constexpr int kChainThreads = 64 * (2 + kStagers);  // + the solving wave and the writer wave
 constexpr int kWriteChunk = 16;           // windows the writer copies to HBM at a time
@@ -49,9 +49,11 @@ constexpr int kWriteChunk = 16;           // windows the writer copies to HBM at
 // Byte layout of one window's record for value type T; lane L = l + 32 h:
 //   A     T[8][64][2]      A_k[l][16 h + 2 i + t] at ((i * 64 + L) * 2 + t), 0 where the column
 //                          index exceeds l (A_k is lower triangular in lane order)
-//   nval  T[2][64][2]      near entry e of lane L at (((e >> 1) * 64 + L) * 2 + (e & 1)), 0 for
+//   nval  T[3][64][2]      near entry e of lane L at (((e >> 1) * 64 + L) * 2 + (e & 1)), 0 for
 //                          padding
-//   ncol  u16[64][4]       its ring index (column & (kRing - 1)) at (L * 4 + e); kRing for padding
+//   ncol  u16[64][4]       its ring index (column & (kRing - 1)) at (L * 4 + e) for e < 4, kRing
+//                          for padding
+//   ncol2 u16[64][2]       the same for e = 4, 5 at (L * 2 + e - 4)
 //   flags int[4]           [0] = 1: the window takes the direct path
 //   --- the part above is staged into an LDS slot ---
 //   oval  T[kOldSlots][64] old entries, folded into b by the stager
@@ -60,7 +62,8 @@ template <typename T>
 struct RecLayout {
     static constexpr int nval() { return 16 * 64 * (int)sizeof(T); }
     static constexpr int ncol() { return nval() + kNear * 64 * (int)sizeof(T); }
-    static constexpr int flags() { return ncol() + 64 * kNear * 2; }
+    static constexpr int ncol2() { return ncol() + 64 * 4 * 2; }
+    static constexpr int flags() { return ncol2() + 64 * 2 * 2; }
     static constexpr int staged() { return flags() + 16; }
     static constexpr int oval() { return staged(); }
     static constexpr int ocol() { return oval() + kOldSlots * 64 * (int)sizeof(T); }
@@ -69,7 +72,7 @@ struct RecLayout {
 };
 static_assert(RecLayout<double>::staged() % 16 == 0 && RecLayout<double>::bytes() % 16 == 0, "record alignment");
 static_assert(RecLayout<float>::staged() % 16 == 0 && RecLayout<float>::bytes() % 16 == 0, "record alignment");
-static_assert(kNear == 4, "ncol holds four ring indices per lane");
+static_assert(kNear == 6, "ncol and ncol2 hold six ring indices per lane");
 
 // position of A_k[l][j] in the record, in elements of T
 __host__ __device__ constexpr int apos(int l, int j) {
@@ -140,6 +143,7 @@ struct ChainArgs {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // a native 16-byte register piece
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 template <typename T> struct Pair;
 template <> struct Pair<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct Pair<float> { typedef float type __attribute__((ext_vector_type(2))); };
@@ -244,7 +248,7 @@ __device__ __forceinline__ void win_load(const char* slot, const T* slot_s, int 
         w.A[2 * i] = v.x;
         w.A[2 * i + 1] = v.y;
     }
-    // 13 LDS reads in all: issued behind the 8 reads of g, they stay within the 15 that the
+    // 15 LDS reads in all: issued behind the 8 reads of g, they stay within the 15 that the
     // LDS counter can leave outstanding, so the product with g waits for none of them
     const P* nv = reinterpret_cast<const P*>(slot + L::nval());
 #pragma unroll
@@ -254,17 +258,100 @@ __device__ __forceinline__ void win_load(const char* slot, const T* slot_s, int 
         w.nv[2 * i + 1] = v.y;
     }
     const u16x4 c = reinterpret_cast<const u16x4*>(slot + L::ncol())[lane];
+    const u16x2 c2 = reinterpret_cast<const u16x2*>(slot + L::ncol2())[lane];
     w.nc[0] = c.x;
     w.nc[1] = c.y;
     w.nc[2] = c.z;
     w.nc[3] = c.w;
+    w.nc[4] = c2.x;
+    w.nc[5] = c2.y;
     w.s = slot_s[lane];
     w.flags = *reinterpret_cast<const int*>(slot + L::flags());
 }
 
+// One window on the solving wave: g from the record in `cur` (or the CSR), the next window's
+// record into `nxt`, x = A_k g into the ring.  SB: bytes of an LDS slot.
+template <typename T, int SB>
+__device__ __forceinline__ void solve_window(const ChainArgs<T>& a, int k, int lane, int& hi, int& wr,
+                                             const Win<T>& cur, Win<T>& nxt, T* ring, T* gbuf, char* slot,
+                                             T* slot_s, int* slot_full, int* done, int* written, int* published) {
+    using P = typename Pair<T>::type;
+    const int l = lane & (kW - 1), h = lane >> 5;
+    const int r = hi - 1 - l;
+    const bool valid = r >= 0;
+    // whether the next window is staged: read now, looked at after the gather below (its wait
+    // covers this read too)
+    const int sl = (k + 1) % kSlots;
+    int staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // g: this lane's share of b minus the products with columns outside the window
+    T acc = cur.s;
+    if (cur.flags == 0) {
+        T part = T(0);
+#pragma unroll
+        for (int e = 0; e < kNear; ++e) part = fma(cur.nv[e], ring[cur.nc[e]], part);
+        acc -= part;
+    } else if (valid) {
+        // entries beyond the record's capacity: straight from the CSR, alternating between the
+        // row's two lanes; x of old columns (at least 64 windows back) is read from HBM once the
+        // writer has published it
+        while (__hip_atomic_load(published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < k - 63)
+            __builtin_amdgcn_s_sleep(2);
+        T part = T(0);
+        for (int p = a.rowptr[r] + 1 + h; p < a.rowptr[r + 1]; p += 2) {
+            const int c = a.col[p];
+            if (c < hi) continue;
+            const T xv = c < hi + kRing ? ring[c & (kRing - 1)] : from_bits<T>(poll_bits(a.x + c));
+            part = fma(a.val[p], xv, part);
+        }
+        acc -= part;
+    }
+    const T g = half_sum(acc);
+    if (k + 1 < a.nwin) {
+        while (staged != k + 2) {
+            __builtin_amdgcn_s_sleep(1);
+            staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    // The slot's contents were stored before its flag was (the stager's release), and this
+    // wave's LDS reads execute in issue order: the reads below see them.  The empty asm keeps
+    // the compiler from moving them above the flag test.
+    __asm__ volatile("" ::: "memory");
+    T* gk = gbuf + (k & 1) * kW;
+    if (h == 0) gk[l] = g;
+    __builtin_amdgcn_wave_barrier();
+    const P* gb = reinterpret_cast<const P*>(gk + 16 * h);
+    P gv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) gv[i] = gb[i];
+    // the next window's record, read while the broadcast of g is in flight (after the last
+    // window: a read of a slot that is not used)
+    win_load(slot + sl * SB, slot_s + sl * 64, lane, nxt);
+    // x = A_k g: 16 columns per half, four partial sums
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        s0 = fma(cur.A[2 * i], gv[i].x, s0);
+        s1 = fma(cur.A[2 * i + 1], gv[i].y, s1);
+        s2 = fma(cur.A[2 * i + 2], gv[i + 1].x, s2);
+        s3 = fma(cur.A[2 * i + 3], gv[i + 1].y, s3);
+    }
+    const T xv = half_sum((s0 + s1) + (s2 + s3));
+    // the ring entries about to be replaced (window k - 64) must have been copied to HBM
+    if (wr < k - 63) {
+        while ((wr = __hip_atomic_load(written, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < k - 63)
+            __builtin_amdgcn_s_sleep(1);
+    }
+    if (h == 0 && valid) ring[r & (kRing - 1)] = xv;
+    // This wave's LDS accesses complete in issue order, so the ring store and every read of the
+    // window's slot precede the store below: a relaxed store suffices, and the empty asm keeps
+    // the compiler from moving memory accesses across it.
+    __asm__ volatile("" ::: "memory");
+    if (lane == 0) __hip_atomic_store(done, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    hi -= kW;
+}
+
 template <typename T, int NP>
 __global__ __launch_bounds__(kChainThreads, 1) void k_chain(ChainArgs<T> a) {
-    using P = typename Pair<T>::type;
     constexpr int kSlotBytes = NP * 64 * 16;
     __shared__ T ring[kRing + 1];
     __shared__ __align__(16) char slot[kSlots][kSlotBytes];
@@ -319,86 +406,20 @@ __global__ __launch_bounds__(kChainThreads, 1) void k_chain(ChainArgs<T> a) {
         return;
     }
 
-    // ---- wave 0: the windows in order -------------------------------------------------------
-    const int l = lane & (kW - 1), h = lane >> 5;
+    // ---- wave 0: the windows in order, two per loop pass so that the two register sets of
+    // window records alternate roles without copies ---------------------------------------------
     while (__hip_atomic_load(&slot_full[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 1)
         __builtin_amdgcn_s_sleep(1);
-    Win<T> cur, nxt;
-    win_load(slot[0], slot_s[0], lane, cur);
+    Win<T> w0, w1;
+    win_load(slot[0], slot_s[0], lane, w0);
     int wr = 0;  // last value of `written` seen
     int hi = a.n;
-    for (int k = 0; k < a.nwin; ++k) {
-        const int r = hi - 1 - l;
-        const bool valid = r >= 0;
-        // whether the next window is staged: read now, looked at after the gather below (its
-        // wait covers this read too)
-        const int sl = (k + 1) % kSlots;
-        int staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        // g: this lane's share of b minus the products with columns outside the window
-        T acc = cur.s;
-        if (cur.flags == 0) {
-            T part = T(0);
-#pragma unroll
-            for (int e = 0; e < kNear; ++e) part = fma(cur.nv[e], ring[cur.nc[e]], part);
-            acc -= part;
-        } else if (valid) {
-            // entries beyond the record's capacity: straight from the CSR, alternating between
-            // the row's two lanes; x of old columns (at least 64 windows back) is read from HBM
-            // once the writer has published it
-            while (__hip_atomic_load(&published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < k - 63)
-                __builtin_amdgcn_s_sleep(2);
-            T part = T(0);
-            for (int p = a.rowptr[r] + 1 + h; p < a.rowptr[r + 1]; p += 2) {
-                const int c = a.col[p];
-                if (c < hi) continue;
-                const T xv = c < hi + kRing ? ring[c & (kRing - 1)] : from_bits<T>(poll_bits(a.x + c));
-                part = fma(a.val[p], xv, part);
-            }
-            acc -= part;
-        }
-        const T g = half_sum(acc);
-        if (k + 1 < a.nwin) {
-            while (staged != k + 2) {
-                __builtin_amdgcn_s_sleep(1);
-                staged = __hip_atomic_load(&slot_full[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-        // The slot's contents were stored before its flag was (the stager's release), and this
-        // wave's LDS reads execute in issue order: the reads below see them.  The empty asm keeps
-        // the compiler from moving them above the flag test.
-        __asm__ volatile("" ::: "memory");
-        if (h == 0) gbuf[k & 1][l] = g;
-        __builtin_amdgcn_wave_barrier();
-        const P* gb = reinterpret_cast<const P*>(gbuf[k & 1] + 16 * h);
-        P gv[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) gv[i] = gb[i];
-        // the next window's record, read while the broadcast of g is in flight (after the last
-        // window: a read of a slot that is not used)
-        win_load(slot[sl], slot_s[sl], lane, nxt);
-        // x = A_k g: 16 columns per half, four partial sums
-        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-            s0 = fma(cur.A[2 * i], gv[i].x, s0);
-            s1 = fma(cur.A[2 * i + 1], gv[i].y, s1);
-            s2 = fma(cur.A[2 * i + 2], gv[i + 1].x, s2);
-            s3 = fma(cur.A[2 * i + 3], gv[i + 1].y, s3);
-        }
-        const T xv = half_sum((s0 + s1) + (s2 + s3));
-        // the ring entries about to be replaced (window k - 64) must have been copied to HBM
-        if (wr < k - 63) {
-            while ((wr = __hip_atomic_load(&written, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < k - 63)
-                __builtin_amdgcn_s_sleep(1);
-        }
-        if (h == 0 && valid) ring[r & (kRing - 1)] = xv;
-        // This wave's LDS accesses complete in issue order, so the ring store and every read of
-        // the window's slot precede the store below: a relaxed store suffices, and the empty asm
-        // keeps the compiler from moving memory accesses across it.
-        __asm__ volatile("" ::: "memory");
-        if (lane == 0) __hip_atomic_store(&done, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        cur = nxt;
-        hi -= kW;
+    for (int k = 0; k < a.nwin; k += 2) {
+        solve_window<T, kSlotBytes>(a, k, lane, hi, wr, w0, w1, ring, &gbuf[0][0], &slot[0][0], &slot_s[0][0],
+                                    slot_full, &done, &written, &published);
+        if (k + 1 >= a.nwin) break;
+        solve_window<T, kSlotBytes>(a, k + 1, lane, hi, wr, w1, w0, ring, &gbuf[0][0], &slot[0][0], &slot_s[0][0],
+                                    slot_full, &done, &written, &published);
     }
 }
 
@@ -420,10 +441,13 @@ int build_records(sptrsv_plan_s* P, char** out) {
         const int hi = n - kW * k, lo = std::max(0, hi - kW);
         T* nv = reinterpret_cast<T*>(R + L::nval());
         unsigned short* nc = reinterpret_cast<unsigned short*>(R + L::ncol());
+        unsigned short* nc2 = reinterpret_cast<unsigned short*>(R + L::ncol2());
+        auto ncref = [&](int e, int lane) -> unsigned short& { return e < 4 ? nc[lane * 4 + e] : nc2[lane * 2 + e - 4]; };
         int* flags = reinterpret_cast<int*>(R + L::flags());
         T* ov = reinterpret_cast<T*>(R + L::oval());
         int* oc = reinterpret_cast<int*>(R + L::ocol());
-        for (int q = 0; q < kNear * 64; ++q) nc[q] = (unsigned short)kRing;
+        for (int q = 0; q < 64; ++q)
+            for (int e = 0; e < kNear; ++e) ncref(e, q) = (unsigned short)kRing;
         for (int q = 0; q < kOldSlots * 64; ++q) oc[q] = -1;
         auto npos = [](int e, int lane) { return (((e >> 1) * 64 + lane) * 2) + (e & 1); };
         bool direct = false;
@@ -441,7 +465,7 @@ int build_records(sptrsv_plan_s* P, char** out) {
                         break;
                     }
                     nv[npos(e, L2)] = (T)val[(size_t)p];
-                    nc[L2 * 4 + e] = (unsigned short)(c & (kRing - 1));
+                    ncref(e, L2) = (unsigned short)(c & (kRing - 1));
                     ++ne;
                 } else {
                     const int q = no >> 1, L2 = l + 32 * (no & 1);
@@ -456,10 +480,11 @@ int build_records(sptrsv_plan_s* P, char** out) {
             }
         }
         if (direct) {  // the solving wave reads the CSR for this window: no near or old entries
-            for (int q = 0; q < kNear * 64; ++q) {
-                nv[q] = T(0);
-                nc[q] = (unsigned short)kRing;
-            }
+            for (int q = 0; q < 64; ++q)
+                for (int e = 0; e < kNear; ++e) {
+                    nv[npos(e, q)] = T(0);
+                    ncref(e, q) = (unsigned short)kRing;
+                }
             for (int q = 0; q < kOldSlots * 64; ++q) {
                 ov[q] = T(0);
                 oc[q] = -1;

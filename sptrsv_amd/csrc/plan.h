@@ -13,6 +13,10 @@ struct sptrsv_plan_s {
     int64_t nnz = 0;
     int n_levels = 0;
     int max_row_len = 0;
+    double analysis_ms = 0.0;  // host wall time of analyse()
+    double levels_ms = 0.0;    // of its level computation
+    int depth_lb = 0;   // depth of the forest "row -> its first off-diagonal column": a lower
+                        // bound on the level count, found on the host at plan creation
     bool grid = false;  // off-diagonal offsets of a natural-order 2D/3D grid factor (block.hip)
 
     // CSR, natural row order, diagonal first (HBM: 4(n+1) + 12 nnz bytes)
@@ -145,6 +149,7 @@ struct sptrsv_plan_s {
 
 namespace sptrsv {
 int analyse(sptrsv_plan_s* p);           // analysis.hip
+int chain_levels(sptrsv_plan_s* p, int* ctrl);  // chain_levels.hip: levels of a chain-like factor
 int ensure_fp32(sptrsv_plan_s* p);       // analysis.hip
 int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, const double* val,
                      int* colptr, int* rowidx, double* cval, hipStream_t s);  // analysis.hip

@@ -1,6 +1,7 @@
 // runtime.hip -- device/memory C ABI, input validation, plan lifecycle and timing.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -274,6 +275,17 @@ int sptrsv_plan_create_csr(int n, int64_t nnz, const int64_t* rowptr, const int*
     for (int i = 0; i <= n; ++i) rp32[i] = (int)rowptr[i];
     for (int i = 0; i < n; ++i) max_len = std::max(max_len, rp32[i + 1] - rp32[i]);
     p->max_row_len = max_len;
+    {  // every row's first off-diagonal column is a dependency: the depth of that forest bounds
+       // the level count from below (one pass, rows in descending order)
+        std::vector<int> depth((size_t)n, 1);
+        int deepest = n > 0 ? 1 : 0;
+        for (int r = n - 1; r >= 0; --r)
+            if (rp32[r + 1] - rp32[r] > 1) {
+                depth[r] = 1 + depth[col[rp32[r] + 1]];
+                deepest = std::max(deepest, depth[r]);
+            }
+        p->depth_lb = deepest;
+    }
     int64_t dims[3];
     p->grid = detect_grid(n, rp32.data(), col, dims);
     hipError_t e = hipMemcpy(p->rowptr, rp32.data(), ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice);
@@ -283,7 +295,9 @@ int sptrsv_plan_create_csr(int n, int64_t nnz, const int64_t* rowptr, const int*
         sptrsv_plan_destroy(p);
         return fail(SPTRSV_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
     }
+    const auto t0 = std::chrono::steady_clock::now();
     rc = analyse(p);
+    p->analysis_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc != SPTRSV_OK) { sptrsv_plan_destroy(p); return rc; }
     *out = p;
     return SPTRSV_OK;
@@ -319,6 +333,13 @@ int sptrsv_plan_info(sptrsv_plan_t p, int* n, int64_t* nnz, int* n_levels) {
     if (n) *n = p->n;
     if (nnz) *nnz = p->nnz;
     if (n_levels) *n_levels = p->n_levels;
+    return SPTRSV_OK;
+}
+
+int sptrsv_plan_analysis_time(sptrsv_plan_t p, double* analysis_ms, double* levels_ms) {
+    if (!p) return fail(SPTRSV_ERR_BAD_INPUT, "null plan");
+    if (analysis_ms) *analysis_ms = p->analysis_ms;
+    if (levels_ms) *levels_ms = p->levels_ms;
     return SPTRSV_OK;
 }
 

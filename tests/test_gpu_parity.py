@@ -135,6 +135,8 @@ def test_chain_window_and_ring_edges(n):
     o = np.lexsort((cols, rows))
     U = gen._from_rows(n, rows[o], cols[o], vals[o], "edges")
     with capi.Plan.from_upper(U) as P:
+        lv, L = O.levels(U.rowptr, U.col)  # n >= 2048: levels from the windowed sweep
+        assert P.n_levels == L and np.array_equal(P.levels(), lv.astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=1)
         x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))
         assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL
@@ -159,6 +161,8 @@ def test_chain_direct_path_windows():
     o = np.lexsort((cols, rows))
     U = gen._from_rows(n, rows[o], cols[o], vals[o], "direct")
     with capi.Plan.from_upper(U) as P:
+        lv, L = O.levels(U.rowptr, U.col)  # windowed level sweep with direct windows and old columns
+        assert P.n_levels == L and np.array_equal(P.levels(), lv.astype(np.int32))
         xt, b = gen.rhs_for(U, 1, seed=3)
         ref = oracle_solve(U, b)
         x = P.solve_host(b, capi.params(variant=capi.VAR_CHAIN))

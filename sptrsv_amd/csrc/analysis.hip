@@ -55,74 +55,79 @@ __global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict
     }
 }
 
-// Level of every row by a sync-free max-plus sweep: rows are taken in descending order by
-// wave tickets; a row's level is 1 + the deepest level among its off-diagonal columns (1 for a
-// diagonal-only row), exactly the reference's rule.  level 0 marks "not yet known".  Each lane
-// walks its row from the far end (the largest columns belong to rows earlier in the sweep, which
-// are known first), kLevelWin entries at a time: their columns are read once per window and
-// their levels are polled together, so a row waits one device round trip after its last
-// dependency is known, not one per entry.
+// Level of every row by a sync-free max-plus sweep: waves take 64-row tickets in descending row
+// order, loop after loop (persistent: a finished wave takes the next ticket instead of making way
+// for a new wave); a row's level is 1 + the deepest level among its off-diagonal columns (1 for
+// a diagonal-only row), exactly the reference's rule (src/core.cpp:244-270).  level 0 marks "not
+// yet known".  Each lane walks its row from the far end (the largest columns belong to rows
+// earlier in the sweep, which are known first), up to kLevelWin entries at a time: their columns
+// are read once per window and their levels are polled together, so a row waits one device round
+// trip after its last dependency is known, not one per entry.  A wave only ever waits on rows of
+// smaller tickets, held by waves that are running: the sweep cannot deadlock.
 constexpr int kLevelWin = 8;
 
 __global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ rowptr,
                                                 const int* __restrict__ col, int* levels,
                                                 int* ctrl, uint64_t limit) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int t = wave_ticket(&ctrl[0]);
-    const int row = n - 1 - (t * kWave + lane);
-    bool done = row < 0;
-    int first = 0, p = 0, deepest = 0, mine = 0;
-    bool has = false;
-    if (!done) {
-        first = rowptr[row] + 1;
-        p = rowptr[row + 1];
-        has = p > first;
-    }
-    int c[kLevelWin];
-    bool need_cols = true;
+    int mine = 0;
     Watchdog wd;
     wd.arm(&ctrl[1], limit);
-    int idle = 0;
     while (true) {
-        bool progressed = false;
-        if (!done && p > first) {
-            // window [q0, p): entry j is p - 1 - j; positions below q0 repeat q0 (harmless)
-            const int q0 = max(first, p - kLevelWin);
-            if (need_cols) {
+        const int t = wave_ticket(&ctrl[0]);
+        if (t * kWave >= n) break;
+        const int row = n - 1 - (t * kWave + lane);
+        bool done = row < 0;
+        int first = 0, p = 0, deepest = 0;
+        bool has = false;
+        if (!done) {
+            first = rowptr[row] + 1;
+            p = rowptr[row + 1];
+            has = p > first;
+        }
+        int c[kLevelWin];
+        int cnt = 0;  // entries in the current window [p - cnt, p); 0: columns not read yet
+        int idle = 0;
+        while (true) {
+            bool progressed = false;
+            if (!done && p > first) {
+                if (cnt == 0) {
+                    cnt = min(kLevelWin, p - first);
 #pragma unroll
-                for (int j = 0; j < kLevelWin; ++j) c[j] = col[max(q0, p - 1 - j)];
-                need_cols = false;
+                    for (int j = 0; j < kLevelWin; ++j) c[j] = col[max(p - cnt, p - 1 - j)];
+                }
+                int lv[kLevelWin];
+#pragma unroll
+                for (int j = 0; j < kLevelWin; ++j)
+                    lv[j] = j < cnt ? __hip_atomic_load(&levels[c[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
+                bool all = true;
+                int mx = deepest;
+#pragma unroll
+                for (int j = 0; j < kLevelWin; ++j) {
+                    all = all && lv[j] != 0;
+                    mx = lv[j] > mx ? lv[j] : mx;
+                }
+                if (all) {
+                    deepest = mx;
+                    p -= cnt;
+                    cnt = 0;
+                    progressed = true;
+                }
             }
-            int lv[kLevelWin];
-#pragma unroll
-            for (int j = 0; j < kLevelWin; ++j)
-                lv[j] = __hip_atomic_load(&levels[c[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bool all = true;
-            int mx = deepest;
-#pragma unroll
-            for (int j = 0; j < kLevelWin; ++j) {
-                all = all && lv[j] != 0;
-                mx = lv[j] > mx ? lv[j] : mx;
-            }
-            if (all) {
-                deepest = mx;
-                p = q0;
-                need_cols = true;
+            if (!done && p <= first) {
+                const int lev = has ? 1 + deepest : 1;
+                __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                mine = lev > mine ? lev : mine;
+                done = true;
                 progressed = true;
             }
+            if (__all(done)) break;
+            if (!__any(progressed)) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
+            }
         }
-        if (!done && p <= first) {
-            const int lev = has ? 1 + deepest : 1;
-            __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            mine = lev;
-            done = true;
-            progressed = true;
-        }
-        if (__all(done)) break;
-        if (!__any(progressed)) {
-            __builtin_amdgcn_s_sleep(1);
-            if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
-        }
+        if (__hip_atomic_load(&ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;  // watchdog
     }
     // one atomic per wave for the maximum level
 #pragma unroll
@@ -250,7 +255,9 @@ int analyse(sptrsv_plan_s* p) {
     } else {
         const int waves = (n + kWave - 1) / kWave;
         const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
-        k_levels<<<(waves + 3) / 4, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
+        // persistent waves: as many as the device holds at once, or fewer for small factors
+        const int blocks = std::min((waves + 3) / 4, device_compute_units() * 8);
+        k_levels<<<blocks, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
     }
     int ctrl[3] = {0, 0, 0};
     SPT_HIP(hipMemcpyAsync(ctrl, p->ticket, sizeof(ctrl), hipMemcpyDeviceToHost, s));

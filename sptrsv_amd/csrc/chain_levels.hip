@@ -13,8 +13,9 @@
 // all windows in parallel), with each lane's near columns (within 2048 rows above the window:
 // their levels are in an LDS ring) and old columns (further up: read back from the levels
 // array).  The solving wave loads each window's record kLAhead windows ahead and touches LDS
-// only for the ring and the broadcast of e.  Levels are integers: the result is exactly the
-// sweep's.
+// only for the ring and the broadcast of e; a second wave reads every record's lines about
+// kLTouch windows ahead of it, so that those loads hit L2.  Levels are integers: the result is
+// exactly the sweep's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +32,7 @@ constexpr int kLRing = 2048;       // levels kept in LDS; ring[kLRing] is a cons
 constexpr int kLNear = 6;          // near columns per lane (12 per row)
 constexpr int kLOld = 2;           // old columns per lane (4 per row)
 constexpr int kLAhead = 6;         // windows between a record's load and its use
+constexpr int kLTouch = 48;        // windows between a record's first read (into L2) and its use
 constexpr unsigned kLDirect = 0xFFFF;  // nc[0] of every lane of a window on the direct path
 constexpr int kNoPath = -(1 << 30);
 
@@ -211,14 +213,30 @@ __device__ __forceinline__ void level_window(int n, int nwin, int k, int lane, c
     half_old = old_levels(levels, half);
 }
 
-__global__ __launch_bounds__(64) void k_chain_levels(int n, int nwin, const int* __restrict__ rowptr,
-                                                     const int* __restrict__ col, const char* rec, int* levels,
-                                                     int* ctrl) {
+__global__ __launch_bounds__(128) void k_chain_levels(int n, int nwin, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, const char* rec, int* levels,
+                                                      int* ctrl) {
     __shared__ int ring[kLRing + 1];
     __shared__ __align__(16) int gbuf[2][kLW];
-    const int lane = threadIdx.x;
-    if (lane == 0) ring[kLRing] = 0;  // what padding entries read
+    __shared__ int done;  // windows finished by the solving wave
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (tid == 0) {
+        ring[kLRing] = 0;  // what padding entries read
+        done = 0;
+    }
     __syncthreads();
+    if (wave == 1) {
+        // one word of every 128-byte line of each record, kLTouch windows ahead of the solving wave
+        unsigned touch = 0;
+        for (int w = 0; w < nwin; ++w) {
+            while (w > __hip_atomic_load(&done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) + kLTouch)
+                __builtin_amdgcn_s_sleep(4);
+            const unsigned* R = reinterpret_cast<const unsigned*>(rec + (size_t)w * kLRecBytes);
+            touch |= R[min(lane * 32, kLRecBytes / 4 - 1)];  // 64 lanes x 128 bytes >= a record
+        }
+        __asm__ volatile("" ::"v"(touch));  // a use of the words read: the reads stay
+        return;
+    }
     static_assert(kLAhead == 6, "the loop below rotates six record sets");
     LWin w[6];
     int o[6];
@@ -231,9 +249,11 @@ __global__ __launch_bounds__(64) void k_chain_levels(int n, int nwin, const int*
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int k = k0 + q;
-            if (k < nwin)
+            if (k < nwin) {
                 level_window(n, nwin, k, lane, rowptr, col, rec, levels, ring, &gbuf[0][0], w[q], o[q], w[q],
                              w[(q + 3) % 6], o[(q + 3) % 6], deepest);
+                if (lane == 0) __hip_atomic_store(&done, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     }
 #pragma unroll
@@ -253,7 +273,7 @@ int chain_levels(sptrsv_plan_s* p, int* ctrl) {
     char* rec = nullptr;
     SPT_HIP(hipMalloc((void**)&rec, (size_t)nwin * kLRecBytes));
     k_level_records<<<nwin, 64, 0, p->stream>>>(n, nwin, p->rowptr, p->col, rec);
-    k_chain_levels<<<1, 64, 0, p->stream>>>(n, nwin, p->rowptr, p->col, rec, p->levels, ctrl);
+    k_chain_levels<<<1, 128, 0, p->stream>>>(n, nwin, p->rowptr, p->col, rec, p->levels, ctrl);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     (void)hipFree(rec);

@@ -247,7 +247,57 @@ struct PushArgs {
     uint64_t limit;
 };
 
-// One wave per CSC column, descending by ticket (the reference: src/kernel_full.cl:54-228).
+// The reference's sync-free algorithm (src/kernel_full.cl:54-228): column j waits until its
+// in-degree counter reads 1 (only its diagonal is left), forms x_j = (b_j - left_sum_j) / diag,
+// adds val * x_j to the left_sum of every row it feeds and decrements their counters.  No
+// fences: left_sum and the counters are device-scope atomics (performed past the XCDs' L2s), a
+// column reads its own left_sum with a device-scope load once its counter reads 1, and a
+// column's additions are waited for (vmcnt 0: performed) before its decrements are issued.
+//
+// m = 1: one lane per column.  Persistent waves take tickets of 64 columns in descending order;
+// a wave only waits on columns of smaller tickets, held by running waves.
+template <typename T>
+__global__ __launch_bounds__(256) void k_push1(PushArgs<T> a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    while (true) {
+        const int t = wave_ticket(&a.ctrl[0]);
+        if (t * kWave >= a.n) break;
+        const int j = a.n - 1 - (t * kWave + lane);
+        bool done = j < 0;
+        int cs = 0, ce = 0;
+        T diag = T(1);
+        if (!done) {
+            cs = a.colptr[j];
+            ce = a.colptr[j + 1] - 1;
+            diag = a.val[ce];
+        }
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        int idle = 0;
+        while (true) {
+            const bool ready = !done && __hip_atomic_load(&a.indeg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1;
+            if (ready) {
+                const T xj = (a.b[j] - from_bits<T>(poll_bits(a.left_sum + j))) / diag;
+                a.x[j] = xj;
+                for (int e = cs; e < ce; ++e) atomicAdd(&a.left_sum[a.rowidx[e]], a.val[e] * xj);
+            }
+            wait_vmem();  // this round's additions are performed
+            if (ready) {
+                for (int e = cs; e < ce; ++e)
+                    __hip_atomic_fetch_add(&a.indeg[a.rowidx[e]], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                done = true;
+            }
+            if (__all(done)) break;
+            if (!__any(ready)) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++idle & 63) == 0 && !done && wd.expired()) done = true;
+            }
+        }
+        if (__hip_atomic_load(&a.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;  // watchdog
+    }
+}
+
+// One wave per column, descending by ticket; its lanes take the right-hand sides.
 template <typename T>
 __global__ __launch_bounds__(64) void k_push(PushArgs<T> a) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -257,8 +307,6 @@ __global__ __launch_bounds__(64) void k_push(PushArgs<T> a) {
     const int cs = a.colptr[j], ce = a.colptr[j + 1] - 1;
     const T diag = a.val[ce];
     if (lane == 0) {
-        // relaxed polls (an acquire load invalidates the L2 on every iteration); the single
-        // acquire fence below orders the column's reads after the last decrement it saw
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
         int idle = 0;
@@ -267,32 +315,41 @@ __global__ __launch_bounds__(64) void k_push(PushArgs<T> a) {
             if ((++idle & 63) == 0 && wd.expired()) break;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_wave_barrier();
     const size_t m = (size_t)a.m;
-    if (a.m == 1) {
+    if (a.m == 1) {  // one value: lane 0 forms it, the lanes share the column's entries
         T xj = T(0);
         if (lane == 0) {
-            xj = (a.b[j] - a.left_sum[j]) / diag;
+            xj = (a.b[j] - from_bits<T>(poll_bits(a.left_sum + j))) / diag;
             a.x[j] = xj;
         }
         xj = __shfl(xj, 0, kWave);
-        for (int e = cs + lane; e < ce; e += kWave)
-            atomicAdd(&a.left_sum[a.rowidx[e]], a.val[e] * xj);
-    } else {
-        for (int k = lane; k < a.m; k += kWave) {
-            const size_t at = (size_t)j * m + k;
-            a.x[at] = (a.b[at] - a.left_sum[at]) / diag;
-        }
-        for (int e = cs; e < ce; ++e) {
-            const int r = a.rowidx[e];
-            const T v = a.val[e];
-            for (int k = lane; k < a.m; k += kWave)
-                atomicAdd(&a.left_sum[(size_t)r * m + k], v * a.x[(size_t)j * m + k]);
+        for (int e = cs + lane; e < ce; e += kWave) atomicAdd(&a.left_sum[a.rowidx[e]], a.val[e] * xj);
+    } else {  // lane k holds x_j of right-hand side k (64 at a time) in a register
+        for (int k0 = 0; k0 < a.m; k0 += kWave) {
+            const int k = k0 + lane;
+            if (k < a.m) {
+                const size_t at = (size_t)j * m + k;
+                const T xk = (a.b[at] - from_bits<T>(poll_bits(a.left_sum + at))) / diag;
+                a.x[at] = xk;
+                for (int e = cs; e < ce; ++e) atomicAdd(&a.left_sum[(size_t)a.rowidx[e] * m + k], a.val[e] * xk);
+            }
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wait_vmem();  // the column's additions are performed before its decrements
     for (int e = cs + lane; e < ce; e += kWave)
         __hip_atomic_fetch_add(&a.indeg[a.rowidx[e]], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// launch of the push solve: lane per column at m = 1 on factors with wide levels, wave per
+// column otherwise (on a near-chain a wave's 64 columns would resolve one after another, each
+// round paying its own wait for the additions)
+template <typename T>
+void launch_push(const PushArgs<T>& a, hipStream_t s, int n_levels) {
+    if (a.m == 1 && (int64_t)n_levels * 4 <= a.n)
+        k_push1<T><<<std::max(1, std::min((a.n + 255) / 256, device_compute_units() * 8)), 256, 0, s>>>(a);
+    else
+        k_push<T><<<a.n, 64, 0, s>>>(a);
 }
 
 // Bulk-synchronous level step: rows [lo, hi) of the level-ordered CSR, G lanes per row over
@@ -604,7 +661,7 @@ int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, int occ, i
             PushArgs<T> a{n, m, P->colptr, P->rowidx, cval, P->indeg_work, ls, d_b, d_x, P->ticket, limit};
             hipEvent_t stop;
             SPT_CHECK(begin_timed(P, &stop));
-            k_push<T><<<n, 64, 0, s>>>(a);
+            launch_push<T>(a, s, P->n_levels);
             end_timed(P, stop);
             SPT_HIP(hipGetLastError());
             return SPTRSV_OK;
@@ -790,7 +847,7 @@ int sptrsv_solve_push_noalloc(sptrsv_plan_t P, const double* d_b, double* d_x, i
     k_copy_int<<<grid_for(n), 256, 0, s>>>(P->indeg, indeg, n);
     k_reset_ctrl<<<1, 64, 0, s>>>(ctrl);
     PushArgs<double> a{P->n, m, P->colptr, P->rowidx, P->cval, indeg, ls, d_b, d_x, ctrl, limit};
-    k_push<double><<<P->n, 64, 0, s>>>(a);
+    launch_push<double>(a, s, P->n_levels);
     k_carry_status<<<1, 64, 0, s>>>(ctrl, P->ticket);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

@@ -270,21 +270,6 @@ int upload(X** dst, const std::vector<X>& src) {
     return SPTRSV_OK;
 }
 
-// r = b - U x over the batch (FP64 products and sums); r32 rounds it once to FP32
-template <typename R>
-__global__ __launch_bounds__(256) void k_batch_residual(int n, int m, const int64_t* __restrict__ rowptr,
-                                                        const int* __restrict__ col, const double* __restrict__ val,
-                                                        const double* __restrict__ x, const double* __restrict__ b,
-                                                        R* __restrict__ r) {
-    const int64_t total = (int64_t)n * m;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
-        const int row = (int)(q / m), k = (int)(q % m);
-        double acc = 0.0;
-        for (int64_t e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
-        r[q] = (R)(b[q] - acc);
-    }
-}
-
 __global__ void k_val_to_float(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = (float)in[i];
@@ -638,8 +623,7 @@ int sptrsv_batch_solve_refined_ex(sptrsv_batch_t B, const sptrsv_refine_t* opts,
         return launch_batch<float>(B, B->val32, in, out, m, nullptr);
     };
     auto residual = [&](const double* b, const double* x, float* r) -> int {
-        k_batch_residual<float><<<grid_for_b((int64_t)B->n * m), 256, 0, P->stream>>>(B->n, m, B->rowptr, B->col,
-                                                                                      B->val, x, b, r);
+        launch_residual<int64_t, float>(B->n, B->nnz, m, B->rowptr, B->col, B->val, x, b, r, P->stream);
         SPT_HIP(hipGetLastError());
         return SPTRSV_OK;
     };
@@ -678,8 +662,7 @@ int sptrsv_batch_sync(sptrsv_batch_t B) {
 // r = b - U x over every system, FP64, asynchronous on the batch's stream.
 int sptrsv_batch_residual(sptrsv_batch_t B, const double* d_b, const double* d_x, double* d_r, int m) {
     if (!B || !d_b || !d_x || !d_r || m < 1) return fail(SPTRSV_ERR_BAD_INPUT, "bad residual arguments");
-    k_batch_residual<double><<<grid_for_b((int64_t)B->n * m), 256, 0, B->ctl->stream>>>(B->n, m, B->rowptr, B->col,
-                                                                                        B->val, d_x, d_b, d_r);
+    launch_residual<int64_t, double>(B->n, B->nnz, m, B->rowptr, B->col, B->val, d_x, d_b, d_r, B->ctl->stream);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }

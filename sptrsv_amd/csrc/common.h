@@ -1,6 +1,8 @@
 // common.h -- shared host/device helpers of the gfx950 solver library.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include <string>
@@ -113,5 +115,53 @@ struct Watchdog {
         return false;
     }
 };
+
+// r = b - U x (FP64 products and sums, rounded once to R): G lanes per (row, right-hand side),
+// striding over the row's entries (coalesced reads of a row), then a shuffle reduction.
+// Fixed reduction order: the same x gives the same r on every run.
+template <typename IdxT, typename R, int G>
+__global__ __launch_bounds__(256) void k_residual(int n, int m, const IdxT* __restrict__ rowptr,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ x, const double* __restrict__ b,
+                                                  R* __restrict__ r) {
+    const int64_t total = (int64_t)n * m;
+    const int g = threadIdx.x & (G - 1);
+    const int64_t step = (int64_t)gridDim.x * (blockDim.x / G);
+    // every lane of a G-group has the same q: the shuffles below never mix groups
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; q < total; q += step) {
+        const int row = (int)(q / m), k = (int)(q % m);
+        double acc = 0.0;
+        for (IdxT e = rowptr[row] + g; e < rowptr[row + 1]; e += G) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (g == 0) r[q] = (R)(b[q] - acc);
+    }
+}
+
+// lanes per (row, right-hand side) for the residual: about half the mean row length, fewer
+// as the right-hand sides already spread the work
+inline int residual_lanes(int64_t n, int64_t nnz, int m) {
+    const int64_t avg = n > 0 ? nnz / n : 1;
+    int g = 1;
+    while (g < 64 && 2 * g <= avg / (2 * m)) g *= 2;
+    return g;
+}
+
+template <typename IdxT, typename R>
+void launch_residual(int n, int64_t nnz, int m, const IdxT* rowptr, const int* col, const double* val,
+                     const double* x, const double* b, R* r, hipStream_t s) {
+    const int G = residual_lanes(n, nnz, m);
+    const int64_t threads = (int64_t)n * m * G;
+    const int grid = (int)std::min<int64_t>(std::max<int64_t>((threads + 255) / 256, 1), 1 << 20);
+    switch (G) {
+        case 1: k_residual<IdxT, R, 1><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        case 2: k_residual<IdxT, R, 2><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        case 4: k_residual<IdxT, R, 4><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        case 8: k_residual<IdxT, R, 8><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        case 16: k_residual<IdxT, R, 16><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        case 32: k_residual<IdxT, R, 32><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+        default: k_residual<IdxT, R, 64><<<grid, 256, 0, s>>>(n, m, rowptr, col, val, x, b, r); break;
+    }
+}
 
 }  // namespace sptrsv

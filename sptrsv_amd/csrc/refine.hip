@@ -32,20 +32,6 @@ inline int grid_for_r(int64_t count, int block = 256, int cap = 65536) {
     return (int)std::min<int64_t>(std::max<int64_t>(g, 1), cap);
 }
 
-// r32 = (float)(b - U x): FP64 products summed in ascending order, rounded once to FP32
-__global__ __launch_bounds__(256) void k_residual32(int n, int m, const int* __restrict__ rowptr,
-                                                    const int* __restrict__ col, const double* __restrict__ val,
-                                                    const double* __restrict__ x, const double* __restrict__ b,
-                                                    float* __restrict__ r32) {
-    const int64_t total = (int64_t)n * m;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
-        const int row = (int)(q / m), k = (int)(q % m);
-        double acc = 0.0;
-        for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) acc = fma(val[e], x[(size_t)col[e] * m + k], acc);
-        r32[q] = (float)(b[q] - acc);
-    }
-}
-
 // max that keeps NaN: once a NaN is seen it stays (fmax would drop it)
 __device__ __forceinline__ double max_keep_nan(double acc, double v) {
     return (v > acc || v != v) && acc == acc ? v : acc;
@@ -138,7 +124,7 @@ int read_ratio(sptrsv_plan_s* P, int m, std::vector<unsigned long long>& h, doub
 // r = b - U x over the plan's CSR in FP64, stored FP32 (any factor; one thread per (row, rhs)).
 int residual32(sptrsv_plan_s* P, const double* d_b, const double* d_x, float* d_r32, int m) {
     const int64_t nm = (int64_t)P->n * m;
-    k_residual32<<<grid_for_r(nm), 256, 0, P->stream>>>(P->n, m, P->rowptr, P->col, P->val, d_x, d_b, d_r32);
+    launch_residual<int, float>(P->n, P->nnz, m, P->rowptr, P->col, P->val, d_x, d_b, d_r32, P->stream);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }

@@ -225,6 +225,16 @@ def bench_mixed(capi, gen, names, out):
                 P.sync()
                 t64.append(time.perf_counter() - t0)
             x64 = d_x.download()
+            # the fastest FP64 solve the library has for this system (AUTO), for reference
+            prm_auto = capi.params()
+            P.solve(d_b, d_r, m, prm_auto)
+            P.sync()
+            tauto = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                P.solve(d_b, d_r, m, prm_auto)
+                P.sync()
+                tauto.append(time.perf_counter() - t0)
             tpush = None
             if m <= 32:
                 # the plain FP64 sync-free kernel (CSC push, the reference's algorithm) on the same
@@ -246,7 +256,9 @@ def bench_mixed(capi, gen, names, out):
             err = float(np.linalg.norm(xr - x64) / np.linalg.norm(x64))
             out[key] = {"fp64_ms": round(min(t64) * 1e3, 3), "refined_ms": round(min(tref) * 1e3, 3),
                         "speedup": round(min(t64) / min(tref), 3), "iters": it, "error_estimate": ratio,
-                        "rel_err_vs_fp64": err, "low_precision": "bf16+fp32" if prec == capi.BF16 else "fp32"}
+                        "rel_err_vs_fp64": err, "low_precision": "bf16+fp32" if prec == capi.BF16 else "fp32",
+                        "fp64_auto_ms": round(min(tauto) * 1e3, 3),
+                        "fp64_auto_variant": capi.VARIANT_NAMES.get(P.auto_variant(m), "?")}
             if tpush is not None:
                 out[key]["fp64_syncfree_ms"] = round(tpush * 1e3, 3)
                 out[key]["speedup_vs_syncfree"] = round(tpush / min(tref), 3)
@@ -481,12 +493,14 @@ def main():
     extras = {}
     if not args.no_extras:
         runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
-                ("ic0_128", "level_pull", 32), ("ic0_128", "level_pull", 128), ("zipf_1e6", "tile", 1),
+                ("ic0_128", "level_pull", 32), ("ic0_128", "panel", 32), ("ic0_128", "panel", 64),
+                ("ic0_128", "level_pull", 128), ("ic0_128", "panel", 128), ("zipf_1e6", "tile", 1),
                 ("zipf_1e6", "chain", 1), ("zipf_1e6", "auto", 1), ("chol2d_100", "chain", 1),
                 ("chol2d_100", "auto", 1), ("chol2d_100", "level_pull", 1),
                 ("supernodal_4096", "supernodal", 8), ("supernodal_4096", "supernodal", 32),
                 ("supernodal_4096", "supernodal", 128)]
         cache = {args.workload: (U, P)}
+        col_refs = {}  # (workload, m) -> oracle x for several right-hand sides (row-major)
         for wl, vname, mm in runs:
             if (wl, vname, mm) == (args.workload, args.variant, m):
                 continue
@@ -523,7 +537,13 @@ def main():
                     extras[f"{wl}/m{mm}/cpu_supernodal_ms"] = round(t_sn * 1e3, 3)
                     e_err = O.rel_error(np.ascontiguousarray(ref_w.T), xg)
                 else:
-                    e_err = None
+                    # the serial oracle (several right-hand sides: column by column on host
+                    # threads, bitwise the serial solve), once per workload and m
+                    if (wl, mm) not in col_refs:
+                        col_refs[(wl, mm)] = O.base_gauss_upper(UU.rowptr, UU.col, UU.val, bx)[0] if mm == 1 else \
+                            np.ascontiguousarray(O.base_gauss_upper_columns(UU.rowptr, UU.col, UU.val,
+                                                                            np.ascontiguousarray(bx.T)).T)
+                    e_err = O.rel_error(col_refs[(wl, mm)], xg)
                 if vname == "supernodal":
                     extras[f"{wl}/{vname}/m{mm}/TFLOP_s"] = round(supernodal_flops(UU, mm) / (e_kms / 1e3) / 1e12, 3)
                 extras[f"{wl}/{vname}/m{mm}"] = {"ms_per_step": round(e_ms, 4), "kernel_ms": round(e_kms, 4),

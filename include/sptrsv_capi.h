@@ -63,7 +63,13 @@ extern "C" {
                                      per CU (0 = 3 for grid bricks, 4 otherwise); FP64, m <= 4 on
                                      grid factors, m == 1 otherwise; exact reference summation
                                      order */
-#define SPTRSV_VAR_COUNT 9
+#define SPTRSV_VAR_PANEL 9        /* many right-hand sides on a natural-order grid factor: one
+                                     workgroup per (8x8x8 / 16x16 brick, panel of 16 columns of
+                                     X), bricks in dependency order; in-brick dependencies from
+                                     LDS, the upstream bricks' face cells read once after their
+                                     epoch flags; no sentinel fill of x; FP64, any m; exact
+                                     reference summation order */
+#define SPTRSV_VAR_COUNT 10
 
 /* Value precision of the solve. */
 #define SPTRSV_FP64 0
@@ -103,6 +109,11 @@ int sptrsv_free(void* ptr);
 int sptrsv_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int sptrsv_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int sptrsv_memcpy_d2d(void* dst, const void* src, size_t bytes);
+/* Self-test of the kernels' division by a reused reciprocal (no reference counterpart): for
+ * device arrays n, d of `count` doubles, fast[i] = the kernels' n[i] / d[i] with the reciprocal
+ * of d[i] computed once, ieee[i] = the compiled division n[i] / d[i]; the two must be bitwise
+ * equal for every input.  Synchronous. */
+int sptrsv_division_check(const double* n, const double* d, double* fast, double* ieee, int64_t count);
 
 /* ---- host-side validation (no device needed) ---------------------------------------- */
 /* Checks an upper CSR: rowptr[0]==0, monotone, rowptr[n]==nnz, first entry of each row is

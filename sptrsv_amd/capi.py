@@ -33,9 +33,11 @@ VAR_CHAIN = 5
 VAR_SUPERNODAL = 6
 VAR_TILE = 7
 VAR_BLOCK = 8
+VAR_PANEL = 9
 VARIANT_NAMES = {VAR_AUTO: "auto", VAR_CSC_PUSH: "csc_push", VAR_ROW_PULL: "row_pull", VAR_LEVEL_PULL: "level_pull",
                  VAR_LEVELSET: "levelset", VAR_CHAIN: "chain",
-                 VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile", VAR_BLOCK: "block"}
+                 VAR_SUPERNODAL: "supernodal", VAR_TILE: "tile", VAR_BLOCK: "block",
+                 VAR_PANEL: "panel"}
 
 FP64 = 0
 FP32 = 1
@@ -96,6 +98,7 @@ def lib():
         "sptrsv_device_init": (_i, [_i]),
         "sptrsv_device_info": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
         "sptrsv_device_sync": (_i, []),
+        "sptrsv_division_check": (_i, [_vp, _vp, _vp, _vp, ctypes.c_int64]),
         "sptrsv_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
         "sptrsv_free": (_i, [_vp]),
         "sptrsv_memcpy_h2d": (_i, [_vp, _vp, _sz]),
@@ -198,6 +201,20 @@ def device_info():
 
 def device_sync():
     check(lib().sptrsv_device_sync(), "device_sync")
+
+
+def division_check(n: np.ndarray, d: np.ndarray):
+    """the kernels' division with a reused reciprocal and the compiled division, on the device:
+    (fast, ieee), to be bitwise equal"""
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    bufs = [DeviceArray.from_host(n), DeviceArray.from_host(d), DeviceArray(n.shape), DeviceArray(n.shape)]
+    try:
+        check(lib().sptrsv_division_check(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, n.size), "division_check")
+        return bufs[2].download(), bufs[3].download()
+    finally:
+        for b in bufs:
+            b.free()
 
 
 def validate_csr(n, rowptr, col, val) -> int:

@@ -71,6 +71,42 @@ __device__ __forceinline__ T from_bits(typename Bits<T>::U u) {
     return __builtin_bit_cast(T, u);
 }
 
+// Division by a value whose reciprocal is reused, bitwise equal to the compiled IEEE division
+// n / d.  The compiler's FP64 division is: dd = div_scale(d, d, n), r = rcp(dd) refined by two
+// Newton steps, nn = div_scale(n, d, n), q = nn * r, rem = fma(-dd, q, nn),
+// div_fixup(div_fmas(rem, r, q), d, n).  When neither div_scale rescales (dd = d, nn = n, the
+// fmas flag clear) that is div_fixup(fma(fma(-d, n*r, n), r, n*r), d, n) with r depending on d
+// alone: div_recip(d) computes r once, div_by(n, d, r) the rest.  div_scale rescales only near
+// the ends of the exponent range (zero, denormal or huge operands, a quotient exponent beyond
+// about +-768), so div_by takes this path only for biased exponents of n and d in [200, 1800]
+// whose difference is at most 600 -- far inside -- and the ordinary division otherwise.  In
+// that range div_fixup returns its operand (operands and quotient are finite, normal and
+// nonzero, and q + rem * r carries the quotient's sign), so it is left out.
+__device__ __forceinline__ double div_recip(double d) {
+    const double r0 = __builtin_amdgcn_rcp(d);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
+    return __builtin_fma(r1, __builtin_fma(-d, r1, 1.0), r1);
+}
+// The quotient without the fallback; `plain` tells whether it is the IEEE one.
+__device__ __forceinline__ double div_by_plain(double n, double d, double r, bool& plain) {
+#pragma clang fp contract(off)
+    const int en = (int)((__builtin_bit_cast(uint64_t, n) >> 52) & 0x7FF);
+    const int ed = (int)((__builtin_bit_cast(uint64_t, d) >> 52) & 0x7FF);
+    plain = en >= 200 && en <= 1800 && ed >= 200 && ed <= 1800 && en - ed <= 600 && ed - en <= 600;
+    const double q = n * r;
+    return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+__device__ __forceinline__ double div_by(double n, double d, double r) {
+    bool plain;
+    double out = div_by_plain(n, d, r, plain);
+    if (!plain) {
+        double dd = d;
+        asm volatile("" : "+v"(dd));  // a real branch: the compiler would otherwise divide in every lane
+        out = n / dd;
+    }
+    return out;
+}
+
 // Wave-level work ticket: lane 0 takes the next ticket, every lane gets it.  Tickets are
 // handed out in launch order of the waves, so a wave only ever waits on work whose ticket
 // is smaller -- held by a wave that is already resident -- which guarantees progress.

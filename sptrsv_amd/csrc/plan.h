@@ -18,6 +18,7 @@ struct sptrsv_plan_s {
     int depth_lb = 0;   // depth of the forest "row -> its first off-diagonal column": a lower
                         // bound on the level count, found on the host at plan creation
     bool grid = false;  // off-diagonal offsets of a natural-order 2D/3D grid factor (block.hip)
+    int64_t grid_dims[3] = {0, 0, 0};  // its nx, ny, nz
 
     // CSR, natural row order, diagonal first (HBM: 4(n+1) + 12 nnz bytes)
     int* rowptr = nullptr;
@@ -126,6 +127,23 @@ struct sptrsv_plan_s {
     int* blk_step = nullptr;     // per position: 1 + the largest step of its in-block dependencies
     double* blk_val = nullptr;
 
+    // brick panels of the PANEL variant (grid factors, built on first use): bricks in dependency
+    // order, the brick's cell table by step, per row its record (diagonal, entries at +1, +nx,
+    // +nx*ny) and entry mask; per (brick, panel of 16 columns) the epoch of the last solve that
+    // finished it.  pn_tickets: tickets taken so far from the plan's control word 16.
+    bool pn_built = false;
+    int pn_bx = 0, pn_by = 0, pn_bz = 0, pn_gx = 0, pn_gy = 0, pn_gz = 0;
+    int pn_nsteps = 0, pn_nhalo = 0, pn_nbricks = 0, pn_ncells = 0;
+    int* pn_blist = nullptr;
+    int* pn_step_ptr = nullptr;
+    uint64_t* pn_cells = nullptr;
+    double* pn_rec = nullptr;
+    unsigned char* pn_mask = nullptr;
+    int* pn_flag = nullptr;
+    size_t pn_flag_elems = 0;
+    int pn_epoch = 0;
+    unsigned pn_tickets = 0;
+
     // chain variant: one record per 32-row window (packed inverse of its triangle, near and old
     // entries), built on first use; FP64 and FP32 values
     char* chain_rec = nullptr;
@@ -184,11 +202,16 @@ int block_solve(sptrsv_plan_s* p, int precision, int occ, const double* d_b, dou
 bool block_supports(sptrsv_plan_s* p, int m);  // block.hip
 void block_free(sptrsv_plan_s* p);        // block.hip
 bool detect_grid(int n, const int* rowptr, const int* col, int64_t* dims);  // block.hip
+int panel_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
+                uint64_t limit);  // panel.hip
+bool panel_supports(sptrsv_plan_s* p);  // panel.hip
+void panel_free(sptrsv_plan_s* p);      // panel.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
 void chain_free(sptrsv_plan_s* p);                                                        // chain.hip
 int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip
 int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s);  // solve.hip
+int zero_ints_async(int* p, int64_t count, hipStream_t s);  // runtime.hip: in stream order
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);
 void end_timed(sptrsv_plan_s* p, hipEvent_t stop);
 }  // namespace sptrsv

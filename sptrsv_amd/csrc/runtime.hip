@@ -67,6 +67,34 @@ void end_timed(sptrsv_plan_s* p, hipEvent_t stop) {
     if (stop) (void)hipEventRecord(stop, p->stream);
 }
 
+namespace {
+__global__ void k_zero_ints(int* out, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = 0;
+}
+}  // namespace
+
+// Zeroes `count` ints with a kernel on stream `s`, ordered before everything later on that
+// stream.  (hipMemset runs on the null stream, which a non-blocking stream does not wait for,
+// and may return before it has run: the plans' streams need their control words zeroed in
+// stream order.)
+int zero_ints_async(int* p, int64_t count, hipStream_t s) {
+    if (count <= 0) return SPTRSV_OK;
+    const int g = (int)std::min<int64_t>((count + 255) / 256, 65536);
+    k_zero_ints<<<g, 256, 0, s>>>(p, count);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+namespace {
+__global__ void k_division_check(const double* n, const double* d, double* fast, double* ieee, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        fast[i] = div_by(n[i], d[i], div_recip(d[i]));
+        ieee[i] = n[i] / d[i];
+    }
+}
+}  // namespace
+
 // A plan that holds no factor: stream, status word, timing events and the refinement / FP32
 // staging buffers, for solvers that keep their own matrix arrays (the batched solver).
 int plan_create_control(int n, int64_t nnz, sptrsv_plan_s** out) {
@@ -76,8 +104,8 @@ int plan_create_control(int n, int64_t nnz, sptrsv_plan_s** out) {
     p->nnz = nnz;
     *out = p;
     SPT_CHECK(sptrsv_malloc((void**)&p->ticket, 64 * sizeof(int)));
-    SPT_HIP(hipMemset(p->ticket, 0, 64 * sizeof(int)));
     SPT_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    SPT_CHECK(zero_ints_async(p->ticket, 64, p->stream));
     SPT_HIP(hipGetDevice(&p->device));
     p->ticks_per_ms = wallclock_ticks_per_ms();
     return SPTRSV_OK;
@@ -241,6 +269,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     supernodal_free(p);
     tile_free(p);
     block_free(p);
+    panel_free(p);
     chain_free(p);
     void* bufs[] = {p->rowptr, p->col, p->val, p->val32, p->colptr, p->rowidx, p->cval, p->cval32,
                     p->indeg, p->levels, p->level_ptr, p->level_rows, p->lrowptr, p->lcol,
@@ -286,8 +315,9 @@ int sptrsv_plan_create_csr(int n, int64_t nnz, const int64_t* rowptr, const int*
             }
         p->depth_lb = deepest;
     }
-    int64_t dims[3];
+    int64_t dims[3] = {0, 0, 0};
     p->grid = detect_grid(n, rp32.data(), col, dims);
+    if (p->grid) std::copy(dims, dims + 3, p->grid_dims);
     hipError_t e = hipMemcpy(p->rowptr, rp32.data(), ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->col, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->val, val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice);
@@ -431,3 +461,15 @@ int sptrsv_plan_sync(sptrsv_plan_t p) {
 }
 
 }  // extern "C"
+
+extern "C" int sptrsv_division_check(const double* n, const double* d, double* fast, double* ieee, int64_t count) {
+    using namespace sptrsv;
+    if (!n || !d || !fast || !ieee || count < 0) return fail(SPTRSV_ERR_BAD_INPUT, "division check: bad arguments");
+    SPT_CHECK(ensure_device());
+    if (count == 0) return SPTRSV_OK;
+    const int g = (int)std::min<int64_t>((count + 255) / 256, 65536);
+    k_division_check<<<g, 256>>>(n, d, fast, ieee, count);
+    SPT_HIP(hipGetLastError());
+    SPT_HIP(hipDeviceSynchronize());
+    return SPTRSV_OK;
+}

@@ -711,13 +711,16 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // (a dependency chain) goes to the single-wave chain kernel, which keeps each window of rows in
 // LDS instead of paying a device-wide round trip per level; one to three right-hand sides in
 // FP64 over a natural-order grid factor to the block kernel (bricks solved in LDS; with four,
-// its per-step work outgrows the level-ordered pull kernel's); everything else to LEVEL_PULL.
+// its per-step work outgrows the level-ordered pull kernel's); 32 or more in FP64 over a grid
+// factor to the brick-panel kernel (on the 128^3 factor faster per solve than LEVEL_PULL from
+// m = 32 on, slower at 8 and 16); everything else to LEVEL_PULL.
 // With `exact`, only kernels whose FP64 x is bitwise equal to the serial solver qualify.
 int default_variant(sptrsv_plan_s* P, int m, int precision, bool exact) {
     const bool chain_like = (int64_t)P->n_levels * 4 > (int64_t)P->n;
     if (!exact && chain_like && precision == SPTRSV_FP64 && relaxed_supernodes_pay(P)) return SPTRSV_VAR_SUPERNODAL;
     if (!exact && m == 1 && chain_like) return SPTRSV_VAR_CHAIN;
     if (m <= 3 && precision == SPTRSV_FP64 && P->grid && block_supports(P, m)) return SPTRSV_VAR_BLOCK;
+    if (m >= 32 && precision == SPTRSV_FP64 && panel_supports(P)) return SPTRSV_VAR_PANEL;
     return SPTRSV_VAR_LEVEL_PULL;
 }
 
@@ -745,7 +748,8 @@ int solve_fp32_device(sptrsv_plan_s* P, const sptrsv_params_t* prm, const float*
         if (m != 1) return fail(SPTRSV_ERR_UNSUPPORTED, "chain variant solves one right-hand side");
         return chain_solve32(P, d_b32, d_x32);
     }
-    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK || variant == SPTRSV_VAR_SUPERNODAL)
+    if (variant == SPTRSV_VAR_TILE || variant == SPTRSV_VAR_BLOCK || variant == SPTRSV_VAR_SUPERNODAL ||
+        variant == SPTRSV_VAR_PANEL)
         return fail(SPTRSV_ERR_UNSUPPORTED, "this variant has no FP32 form through sptrsv_solve");
     const int unroll = prm->unroll > 0 ? prm->unroll : 4;
     return solve_typed<float>(P, variant, prm->group, unroll, prm->occupancy, prm->block, d_b32, d_x32, m, P->val32,
@@ -805,6 +809,7 @@ int sptrsv_solve(sptrsv_plan_t P, const sptrsv_params_t* params, const double* d
     if (variant == SPTRSV_VAR_SUPERNODAL) return supernodal_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_TILE) return tile_solve(P, prm.precision, prm.block, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_BLOCK) return block_solve(P, prm.precision, prm.occupancy, d_b, d_x, m, limit);
+    if (variant == SPTRSV_VAR_PANEL) return panel_solve(P, prm.precision, d_b, d_x, m, limit);
     if (variant == SPTRSV_VAR_CHAIN && prm.precision == SPTRSV_FP64) return chain_solve(P, prm.precision, d_b, d_x, m);
     if (prm.precision == SPTRSV_FP64)
         return solve_typed<double>(P, variant, prm.group, unroll, prm.occupancy, prm.block, d_b, d_x, m, P->val, P->cval, P->lval, limit);

@@ -622,7 +622,7 @@ int ensure_sn_flags(sptrsv_plan_s* P, int nblk) {
     P->sn_flag = nullptr;
     const size_t count = (size_t)P->sn * nblk;
     SPT_HIP(hipMalloc((void**)&P->sn_flag, count * sizeof(int)));
-    SPT_HIP(hipMemset(P->sn_flag, 0, count * sizeof(int)));
+    SPT_CHECK(zero_ints_async(P->sn_flag, (int64_t)count, P->stream));  // in stream order, before the solve
     P->sn_flag_blocks = nblk;
     return SPTRSV_OK;
 }

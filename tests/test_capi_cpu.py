@@ -234,3 +234,24 @@ def test_tables_layout(tmp_path, monkeypatch):
     assert rows[1:] == ["a.mtx;1.0;4.0;", "b.matrix;0.5;7.0;", "c.txt;2.0;1.0;"]
     assert ("base", "a.mtx", "a.snodes") in calls and ("barrier", "c.txt", "c_snodes.txt") in calls
     assert os.path.exists(tmp_path / "exp" / run / "best_times_log.log")
+
+
+PANEL_EMULATE_CASES = [
+    "16 16 16 16 1 0", "12 12 12 32 2 0.1", "10 9 11 20 3 0.2", "9 6 5 3 6 0.3", "8 8 8 128 7 0",
+    "2 2 2 1 8 0", "3 40 70 17 10 0.05", "17 13 1 16 4 0.1", "33 40 1 5 5 0", "40 3 1 16 9 0",
+]
+
+
+def test_panel_kernel_indexing_emulated(tmp_path):
+    """The PANEL kernel's address arithmetic on the layout tables, emulated on the CPU under
+    AddressSanitizer and UBSan (tests/cpp/panel_emulate.cpp): every dependency read comes from
+    the cell or halo value written before it, no access leaves its buffer, and x is bitwise
+    equal to the serial back substitution -- partial bricks, 2D and 3D grids, missing entries,
+    m not a multiple of the 16-column panel."""
+    exe = str(tmp_path / "panel_emulate")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-I", os.path.join(ROOT, "sptrsv_amd", "csrc"), os.path.join(ROOT, "tests", "cpp", "panel_emulate.cpp"),
+                    "-o", exe], check=True)
+    for case in PANEL_EMULATE_CASES:
+        out = subprocess.run([exe, *case.split()], capture_output=True, text=True)
+        assert out.returncode == 0 and out.stdout.startswith("ok"), (case, out.stdout, out.stderr)

@@ -1,7 +1,7 @@
 """The benchmark configurations at their full sizes on the GPU, against the oracle.
 
 C1  exact Cholesky factor of the 5-point Laplacian on a 100 x 100 grid (n = 10,000, bandwidth 100)
-C2a IC(0) factor of the 7-point Laplacian on a 128^3 grid, m in {32, 128}
+C2a IC(0) factor of the 7-point Laplacian on a 128^3 grid, m in {32, 128} (level pull, brick panels, AUTO)
 C3  random supernodal factor, 4,096 supernodes of widths U[16,128], m in {32, 128}
 Bars: analysis arrays bit-exact (closed forms where they exist), x bitwise equal to the serial
 oracle for the exact-order kernels, relative error <= 1e-12 otherwise."""
@@ -109,8 +109,11 @@ def test_c2a_multi_rhs(c2a, m):
     ref = oracle_rows(U, b)
     x = P.solve_host(b, capi.params(variant=capi.VAR_LEVEL_PULL))
     assert np.array_equal(x, ref)
-    x = P.solve_host(b, capi.params())  # AUTO
+    x = P.solve_host(b, capi.params(variant=capi.VAR_PANEL))
     assert np.array_equal(x, ref)
+    x = P.solve_host(b, capi.params())  # AUTO (brick panels)
+    assert np.array_equal(x, ref)
+    assert P.auto_variant(m) == capi.VAR_PANEL
     assert O.rel_error(xt, ref) < 1e-10
 
 

@@ -191,7 +191,7 @@ def plan_timing(O, U, P, plan_s):
             "levels": P.n_levels, "oracle_host_levels_ms": round(host_ms, 2)}
 
 
-def bench_mixed(capi, gen, names, out):
+def bench_mixed(capi, gen, names, out, only=""):
     """Wall time of the refined solve (reduced-precision solves + FP64 residual and update on the
     GPU, stopped at an estimated relative error <= 1e-13) against the plain FP64 solve of the
     same kernel family and (m <= 32) the plain FP64 sync-free CSC push solve, data resident on
@@ -203,6 +203,8 @@ def bench_mixed(capi, gen, names, out):
     cache = {}
     for wl, vname, m, prec in cases:
         key = f"{wl}/{vname}/m{m}"
+        if only and key != only:
+            continue
         try:
             if wl not in cache:
                 U, _ = make_workload(wl)
@@ -440,6 +442,8 @@ def main():
     ap.add_argument("--suite", default="", help="kappa: only the full condition-number sweep (m in 1, 32, 128); "
                     "batch: only the batched solve at m in 1, 4, 16")
     ap.add_argument("--no-kappa", action="store_true")
+    ap.add_argument("--mixed-case", default="", help="run only this refined-solve case, e.g. "
+                    "supernodal_4096/supernodal/m128 (profiling)")
     args = ap.parse_args()
     t_start = time.perf_counter()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -614,7 +618,7 @@ def main():
     del P
     mixed, batch, sweep = {}, {}, {}
     if not args.no_mixed:
-        bench_mixed(capi, gen, names, mixed)
+        bench_mixed(capi, gen, names, mixed, only=args.mixed_case)
     if not args.no_kappa:
         bench_kappa(capi, gen, sweep, {"zipf_1e6": (1,), "supernodal_4096": (32,)}, targets=(1e2, 1e8))
     if not args.no_batch:

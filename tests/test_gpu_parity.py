@@ -331,8 +331,9 @@ def test_ccs2ccs_pad_device_rejects_bad_partition(plans):
 
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
-    m == 1, natural-order grid factors to the block kernel at m <= 3 in FP64, and everything
-    else to the level-ordered pull kernel (the last two bitwise exact)."""
+    m == 1, natural-order grid factors to the block kernel at m <= 3 in FP64, a plan with a
+    supernodal layout to the supernodal kernel in FP64, and everything else to the
+    level-ordered pull kernel (block and pull bitwise exact)."""
     assert plans["ic0_12"].auto_variant(1) == capi.VAR_BLOCK
     assert plans["ic0_12"].auto_variant(2) == capi.VAR_BLOCK
     assert plans["ic0_12"].auto_variant(1, capi.FP32) == capi.VAR_LEVEL_PULL
@@ -345,6 +346,15 @@ def test_auto_variant(plans):
     assert plans["dense_70"].auto_variant(1) == capi.VAR_SUPERNODAL
     assert plans["dense_70"].auto_variant(4) == capi.VAR_SUPERNODAL
     assert plans["dense_70"].auto_variant(1, capi.FP32) == capi.VAR_CHAIN
+    # a supernodal layout built by the caller: the MFMA kernel in FP64, at every m
+    S = gen.random_supernodal(12, seed=31)
+    with capi.Plan.from_upper(S) as PS:
+        PS.build_supernodes(S.snodes)
+        assert PS.auto_variant(1) == capi.VAR_SUPERNODAL
+        assert PS.auto_variant(8) == capi.VAR_SUPERNODAL
+        assert PS.auto_variant(8, capi.FP32) == capi.VAR_LEVEL_PULL
+        xt, b = gen.rhs_for(S, 8, seed=3, layout="row")
+        assert O.rel_error(oracle_solve(S, b), PS.solve_host(b, capi.params())) <= 1e-12
     for name in MATS:
         U, P = MATS[name], plans[name]
         for m in (1, 2, 3, 4):

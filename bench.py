@@ -497,6 +497,7 @@ def main():
     extras = {}
     if not args.no_extras:
         runs = [("ic0_128", "tile", 1), ("ic0_128", "level_pull", 1), ("ic0_128", "row_pull", 1),
+                ("ic0_128", "levelset", 1), ("ic0_128", "levelset", 32),
                 ("ic0_128", "level_pull", 32), ("ic0_128", "panel", 32), ("ic0_128", "panel", 64),
                 ("ic0_128", "level_pull", 128), ("ic0_128", "panel", 128), ("zipf_1e6", "tile", 1),
                 ("zipf_1e6", "chain", 1), ("zipf_1e6", "auto", 1), ("chol2d_100", "chain", 1),

@@ -144,6 +144,16 @@ struct sptrsv_plan_s {
     int pn_epoch = 0;
     unsigned pn_tickets = 0;
 
+    // chunks of the level-set solve (levelset.hip, built on first use): chunks of level l are
+    // ls_ck_level[l] .. ls_ck_level[l+1]; chunk c is level-ordered slots
+    // [ls_ck_slot[c], ls_ck_slot[c+1]) of level ls_ck_lev[c]; ls_done: per level, chunks
+    // finished in the current solve
+    int* ls_ck_level = nullptr;
+    int* ls_ck_slot = nullptr;
+    int* ls_ck_lev = nullptr;
+    int* ls_done = nullptr;
+    int ls_chunks = 0;
+
     // chain variant: one record per 32-row window (packed inverse of its triangle, near and old
     // entries), built on first use; FP64 and FP32 values
     char* chain_rec = nullptr;
@@ -206,6 +216,9 @@ int panel_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x,
                 uint64_t limit);  // panel.hip
 bool panel_supports(sptrsv_plan_s* p);  // panel.hip
 void panel_free(sptrsv_plan_s* p);      // panel.hip
+template <typename T>
+int levelset_solve(sptrsv_plan_s* p, const T* lval, const T* d_b, T* d_x, int m, uint64_t limit);  // levelset.hip
+void levelset_free(sptrsv_plan_s* p);   // levelset.hip
 int chain_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m);  // chain.hip
 int chain_solve32(sptrsv_plan_s* p, const float* d_b, float* d_x);                        // chain.hip
 void chain_free(sptrsv_plan_s* p);                                                        // chain.hip

@@ -270,6 +270,7 @@ int sptrsv_plan_destroy(sptrsv_plan_t p) {
     tile_free(p);
     block_free(p);
     panel_free(p);
+    levelset_free(p);
     chain_free(p);
     void* bufs[] = {p->rowptr, p->col, p->val, p->val32, p->colptr, p->rowidx, p->cval, p->cval32,
                     p->indeg, p->levels, p->level_ptr, p->level_rows, p->lrowptr, p->lcol,

@@ -15,8 +15,9 @@
 //       (src/kernel_full.cl:54-228): wait for in-degree == 1, x = (b - left_sum) / diag,
 //       scatter x * U(:,j) into left_sum with native FP64 atomics, release, decrement the
 //       in-degree of every touched row.
-//   LEVELSET  k_level_step: bulk-synchronous, one launch per level over the level-ordered
-//       CSR (src/core.cpp:273-376 without the OpenMP task split); plain loads, exact order.
+//   LEVELSET  k_levelset (levelset.hip): bulk-synchronous, one cooperative launch walking the
+//       levels with a grid barrier between them, each level's rows staged in LDS in chunks
+//       (src/core.cpp:273-376 without the OpenMP task split); exact order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -352,29 +353,6 @@ void launch_push(const PushArgs<T>& a, hipStream_t s, int n_levels) {
         k_push<T><<<a.n, 64, 0, s>>>(a);
 }
 
-// Bulk-synchronous level step: rows [lo, hi) of the level-ordered CSR, G lanes per row over
-// the RHS columns.  Every dependency lies in an earlier level, i.e. an earlier launch.
-template <typename T, int G>
-__global__ __launch_bounds__(256) void k_level_step(int lo, int hi, int m, const int* __restrict__ rowptr,
-                                                    const int* __restrict__ col, const T* __restrict__ val,
-                                                    const int* __restrict__ perm, const T* __restrict__ b,
-                                                    T* __restrict__ x) {
-#pragma clang fp contract(off)
-    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
-    const int slot = lo + gid / G;
-    const int gl = gid % G;
-    if (slot >= hi) return;
-    const int r = perm[slot];
-    const int s = rowptr[slot], e = rowptr[slot + 1];
-    const T diag = val[s];
-    for (int k = gl; k < m; k += G) {
-        T acc = T(0);
-        for (int q = s + 1; q < e; ++q) acc = acc + val[q] * x[(size_t)col[q] * m + k];
-        const size_t at = (size_t)r * m + k;
-        x[at] = (b[at] - acc) / diag;
-    }
-}
-
 template <typename T>
 __global__ void k_fill_bits(T* out, typename Bits<T>::U bits, int64_t count) {
     using U = typename Bits<T>::U;
@@ -666,28 +644,8 @@ int solve_typed(sptrsv_plan_s* P, int variant, int group, int unroll, int occ, i
             SPT_HIP(hipGetLastError());
             return SPTRSV_OK;
         }
-        case SPTRSV_VAR_LEVELSET: {
-            const int G = m == 1 ? 1 : std::min(64, next_pow2(m));
-            hipEvent_t stop;
-            SPT_CHECK(begin_timed(P, &stop));
-            for (int l = 0; l < P->n_levels; ++l) {
-                const int lo = P->h_level_ptr[l], hi = P->h_level_ptr[l + 1];
-                const int64_t threads = (int64_t)(hi - lo) * G;
-                const int grid = (int)((threads + 255) / 256);
-                switch (G) {
-                    case 1: k_level_step<T, 1><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    case 2: k_level_step<T, 2><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    case 4: k_level_step<T, 4><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    case 8: k_level_step<T, 8><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    case 16: k_level_step<T, 16><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    case 32: k_level_step<T, 32><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                    default: k_level_step<T, 64><<<grid, 256, 0, s>>>(lo, hi, m, P->lrowptr, P->lcol, lval, P->level_rows, d_b, d_x); break;
-                }
-            }
-            end_timed(P, stop);
-            SPT_HIP(hipGetLastError());
-            return SPTRSV_OK;
-        }
+        case SPTRSV_VAR_LEVELSET:
+            return levelset_solve<T>(P, lval, d_b, d_x, m, limit);
         default:
             return fail(SPTRSV_ERR_UNSUPPORTED, "solver variant " + std::to_string(variant) + " not available");
     }

@@ -2,6 +2,8 @@
 // include/sptrsv_syncfree_opencl.h) on one matrix, the way the reference's `run()` calls them
 // (src/utils.cpp:161-353), and writes each result as raw doubles to <out_dir>/<name>.bin.
 // Needs the GPU.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,6 +15,10 @@
 #include "utils.h"
 
 static void dump(const std::string& dir, const char* name, const double* x, size_t count, double t) {
+    if (t < 0) {  // the entry point reported a failure (negative status instead of a time)
+        fprintf(stderr, "%s failed: %g\n", name, t);
+        exit(4);
+    }
     std::string path = dir + "/" + name + ".bin";
     FILE* f = fopen(path.c_str(), "wb");
     fwrite(x, sizeof(double), count, f);
@@ -42,10 +48,15 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < cnt; ++i) b[i] = 1.0 + (double)((i * 7) % 11);
 
     std::vector<double> xs(cnt);
+    // each output starts as NaN: a call that leaves x untouched cannot pass on an earlier result
+    auto poison = [&]() { std::fill(xs.begin(), xs.end(), std::nan("")); };
+    poison();
     double t = base_gauss_upper(n, val, row, col, xs.data(), b, rhs);
     dump(out, "base", xs.data(), cnt, t);
+    poison();
     t = base_gauss_upper_new(n, val, row, col, xs.data(), b, rhs);
     dump(out, "base_new", xs.data(), cnt, t);
+    poison();
     t = barrier_upper(n, xs.data(), b, val, col, row, 4, rhs);
     dump(out, "barrier", xs.data(), cnt, t);
 
@@ -81,18 +92,23 @@ int main(int argc, char** argv) {
     std::vector<double> brow(cnt), xrow(cnt);
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < rhs; ++k) brow[(size_t)i * rhs + k] = b[i + (size_t)k * n];
+    auto poison_row = [&]() { std::fill(xrow.begin(), xrow.end(), std::nan("")); };
+    poison_row();
     t = sptrsv_syncfree_opencl(col_t, row_t_int.data(), val_t, n, n, (int)nz, xrow.data(), brow.data(), rhs);
     dump(out, "syncfree", xrow.data(), cnt, t);
     if (rhs == 1) {
+        poison_row();
         t = sptrsv_syncfree2_opencl(col_t, row_t_int.data(), val_t, n, n, (int)nz, xrow.data(), brow.data());
         dump(out, "syncfree2", xrow.data(), cnt, t);
         std::vector<int> rp_int((size_t)n + 1);
         for (int i = 0; i <= n; ++i) rp_int[(size_t)i] = (int)row[i];
+        poison_row();
         t = sptrsv_syncfree3_opencl(col, rp_int.data(), val, n, n, (int)nz, xrow.data(), brow.data());
         dump(out, "write_first", xrow.data(), cnt, t);
         // forward substitution on L = U^T (lower CSR, diagonal last)
         std::vector<uint64_t> lrow((size_t)n + 1);
         for (int i = 0; i <= n; ++i) lrow[(size_t)i] = (uint64_t)col_t[i];
+        poison();
         t = base_gauss_lower(n, val_t, lrow.data(), row_t_int.data(), xs.data(), b);
         dump(out, "lower", xs.data(), (size_t)n, t);
     }

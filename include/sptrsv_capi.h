@@ -192,6 +192,18 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
  * factor the choice checks the block layout's limits and so builds that layout (on the host,
  * once per plan; the block solve would build it on its first call). */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
+/* Learned choice of the variant for AUTO (no reference counterpart).  The 24 features of the
+ * plan's factor at m right-hand sides (the batched selector's: log2 n, log2 nnz, row-length
+ * and level-width statistics, offsets, supernodes, log2 m, diagonal dominance); computed once
+ * per plan.  sptrsv_selector_set installs a 24 -> h1 -> h2 -> out ReLU network (parameters in
+ * the sptrsv_mlp_* layout, inputs normalised as (f - mu) / sd, output k scoring variant
+ * classes[k]) that AUTO then consults for every plan: it runs the best-scored variant valid for
+ * the plan, m and precision (exact solves: only the bitwise ones); without a network, or when
+ * no scored variant is valid, AUTO uses its hand rule.  sptrsv_selector_clear removes it. */
+int sptrsv_plan_features(sptrsv_plan_t plan, int m, float* features);
+int sptrsv_selector_set(int in, int h1, int h2, int out, const float* params, const float* mu, const float* sd,
+                        const int* classes);
+int sptrsv_selector_clear(void);
 
 /* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).
  * Asynchronous on the plan's stream. */

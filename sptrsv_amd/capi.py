@@ -136,6 +136,9 @@ def lib():
         "sptrsv_batch_destroy": (_i, [_vp]),
         "sptrsv_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i64)]),
         "sptrsv_batch_features": (_i, [_vp, _i, _vp]),
+        "sptrsv_plan_features": (_i, [_vp, _i, _vp]),
+        "sptrsv_selector_set": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+        "sptrsv_selector_clear": (_i, []),
         "sptrsv_batch_set_configs": (_i, [_vp, _vp, _vp]),
         "sptrsv_batch_solve": (_i, [_vp, _i, _vp, _vp, _i, _i]),
         "sptrsv_batch_sync": (_i, [_vp]),
@@ -201,6 +204,31 @@ def device_info():
 
 def device_sync():
     check(lib().sptrsv_device_sync(), "device_sync")
+
+
+def selector_set(sizes, params, mu, sd, classes):
+    """install a variant-selector network for AUTO (24 -> h1 -> h2 -> out, sptrsv_mlp layout)"""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    mu = np.ascontiguousarray(mu, dtype=np.float32)
+    sd = np.ascontiguousarray(sd, dtype=np.float32)
+    cl = np.ascontiguousarray(classes, dtype=np.int32)
+    i, h1, h2, o = (int(v) for v in sizes)
+    check(lib().sptrsv_selector_set(i, h1, h2, o, _ptr(p), _ptr(mu), _ptr(sd), _ptr(cl)), "selector_set")
+
+
+def selector_clear():
+    check(lib().sptrsv_selector_clear(), "selector_clear")
+
+
+def selector_load(path=None):
+    """install the single-system selector saved by sptrsv_amd.tune_single (default: the one
+    shipped with the package); returns its training summary"""
+    import json
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_single.json")
+    with open(path) as f:
+        doc = json.load(f)
+    selector_set(doc["sizes"], doc["params"], doc["mu"], doc["sd"], doc["classes"])
+    return doc.get("meta", {})
 
 
 def division_check(n: np.ndarray, d: np.ndarray):
@@ -401,6 +429,12 @@ class Plan:
         out = np.zeros(1, dtype=np.int32)
         check(lib().sptrsv_plan_auto_variant(self.handle, m, precision, _ptr(out)), "auto_variant")
         return int(out[0])
+
+    def features(self, m=1):
+        """the 24 selector features of the factor at m right-hand sides"""
+        out = np.zeros(24, dtype=np.float32)
+        check(lib().sptrsv_plan_features(self.handle, m, _ptr(out)), "plan_features")
+        return out
 
     def blocks(self):
         """block order of the BLOCK variant: (kind, block_ptr, perm); kind 1 = grid bricks,

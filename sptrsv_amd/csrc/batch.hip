@@ -298,6 +298,8 @@ int launch_batch(sptrsv_batch_s* B, const T* val, const T* d_b, T* d_x, int m, u
     return SPTRSV_OK;
 }
 
+}  // namespace
+
 // The 24 selector features of one system, computed from its CSR and levels (host).
 void system_features(const int64_t* rp, const int* col, const double* val, const std::vector<int>& lev, int lo, int hi,
                      int m, float* f) {
@@ -389,7 +391,6 @@ void system_features(const int64_t* rp, const int* col, const double* val, const
     f[23] = (float)std::log2((dsum / n) / std::max(osum / nd, 1e-300));
 }
 
-}  // namespace
 }  // namespace sptrsv
 
 using namespace sptrsv;

@@ -18,6 +18,8 @@ struct sptrsv_plan_s {
     int depth_lb = 0;   // depth of the forest "row -> its first off-diagonal column": a lower
                         // bound on the level count, found on the host at plan creation
     bool grid = false;  // off-diagonal offsets of a natural-order 2D/3D grid factor (block.hip)
+    float feat[24] = {};  // selector features (select.hip), computed on first use
+    bool feat_ok = false;
     int64_t grid_dims[3] = {0, 0, 0};  // its nx, ny, nz
 
     // CSR, natural row order, diagonal first (HBM: 4(n+1) + 12 nnz bytes)
@@ -194,6 +196,8 @@ struct RefineResult {
 int refine_loop(sptrsv_plan_s* p, const sptrsv_refine_t& ro, const double* d_b, double* d_x, int m,
                 const std::function<int(const float*, float*)>& solve_low,
                 const std::function<int(const double*, const double*, float*)>& residual, RefineResult* res);
+// AUTO's choice by the installed selector network (select.hip); false: use the hand rule
+bool selector_pick(sptrsv_plan_s* p, int m, int precision, bool exact, int* variant);
 // AUTO's choice (exact: only kernels bitwise equal to the serial solver)
 int default_variant(sptrsv_plan_s* p, int m, int precision, bool exact);  // solve.hip
 // FP32 solve of device b32 -> x32 with the variant in params (not AUTO), m right-hand sides
@@ -225,6 +229,9 @@ void chain_free(sptrsv_plan_s* p);                                              
 int convert_d2f(const double* in, float* out, int64_t count, hipStream_t s);  // solve.hip
 int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s);  // solve.hip
 int zero_ints_async(int* p, int64_t count, hipStream_t s);  // runtime.hip: in stream order
+// the 24 selector features of rows [lo, hi) of a CSR with 1-based levels lev (batch.hip)
+void system_features(const int64_t* rp, const int* col, const double* val, const std::vector<int>& lev, int lo,
+                     int hi, int m, float* f);
 int begin_timed(sptrsv_plan_s* p, hipEvent_t* stop_out);
 void end_timed(sptrsv_plan_s* p, hipEvent_t stop);
 }  // namespace sptrsv

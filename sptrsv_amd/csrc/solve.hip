@@ -665,7 +665,8 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
     return SPTRSV_OK;
 }
 
-// AUTO: one right-hand side over a factor whose levels hold fewer than four rows on average
+// AUTO: the learned selector's choice when a network is installed (sptrsv_selector_set);
+// otherwise this hand rule: one right-hand side over a factor whose levels hold fewer than four rows on average
 // (a dependency chain) goes to the single-wave chain kernel, which keeps each window of rows in
 // LDS instead of paying a device-wide round trip per level; one to three right-hand sides in
 // FP64 over a natural-order grid factor to the block kernel (bricks solved in LDS; with four,
@@ -675,6 +676,8 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // m = 32 on, slower at 8 and 16); everything else to LEVEL_PULL.
 // With `exact`, only kernels whose FP64 x is bitwise equal to the serial solver qualify.
 int default_variant(sptrsv_plan_s* P, int m, int precision, bool exact) {
+    int picked = 0;
+    if (selector_pick(P, m, precision, exact, &picked)) return picked;  // the learned selector
     const bool chain_like = (int64_t)P->n_levels * 4 > (int64_t)P->n;
     // a supernodal layout the caller built (sptrsv_plan_build_supernodes): the MFMA kernel
     if (!exact && P->sn > 0 && P->relaxed_state != 1 && precision == SPTRSV_FP64) return SPTRSV_VAR_SUPERNODAL;

@@ -55,11 +55,10 @@ int plan_features_cached(sptrsv_plan_s* P, int m, float* f) {
 }
 
 // Variants that can run this solve.  SUPERNODAL needs a supernodal layout: one the caller built,
-// or relaxed supernodes on a chain-like factor where the hand rule would build them too (built
-// here on first use, so this is asked only of the best-scored variant).
+// or relaxed supernodes where the hand rule would build them too (built here on first use, so
+// this is asked only of the best-scored variant).
 bool variant_valid(sptrsv_plan_s* P, int v, int m, int precision, bool exact) {
     const bool fp64 = precision == SPTRSV_FP64;
-    const bool chain_like = (int64_t)P->n_levels * 4 > (int64_t)P->n;
     switch (v) {
         case SPTRSV_VAR_LEVEL_PULL:
         case SPTRSV_VAR_ROW_PULL:
@@ -69,7 +68,7 @@ bool variant_valid(sptrsv_plan_s* P, int v, int m, int precision, bool exact) {
         case SPTRSV_VAR_TILE: return fp64 && m == 1;
         case SPTRSV_VAR_BLOCK: return fp64 && (m == 1 || (P->grid && m <= 3 && block_supports(P, m)));
         case SPTRSV_VAR_PANEL: return fp64 && panel_supports(P);
-        case SPTRSV_VAR_SUPERNODAL: return !exact && fp64 && (P->sn > 0 || (chain_like && relaxed_supernodes_pay(P)));
+        case SPTRSV_VAR_SUPERNODAL: return !exact && fp64 && (P->sn > 0 || relaxed_supernodes_pay(P));
         default: return false;
     }
 }

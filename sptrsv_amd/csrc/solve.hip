@@ -671,7 +671,9 @@ int convert_f2d(const float* in, double* out, int64_t count, hipStream_t s) {
 // LDS instead of paying a device-wide round trip per level; one to three right-hand sides in
 // FP64 over a natural-order grid factor to the block kernel (bricks solved in LDS; with four,
 // its per-step work outgrows the level-ordered pull kernel's); a plan with a supernodal layout
-// built by the caller, in FP64, to the supernodal MFMA kernel; 32 or more in FP64 over a grid
+// built by the caller, in FP64, to the supernodal MFMA kernel, and so a factor of 16 or more
+// entries per row whose relaxed supernodes hold 32 rows or more on average (built here once, on
+// first use); 32 or more in FP64 over a grid
 // factor to the brick-panel kernel (on the 128^3 factor faster per solve than LEVEL_PULL from
 // m = 32 on, slower at 8 and 16); everything else to LEVEL_PULL.
 // With `exact`, only kernels whose FP64 x is bitwise equal to the serial solver qualify.
@@ -681,7 +683,7 @@ int default_variant(sptrsv_plan_s* P, int m, int precision, bool exact) {
     const bool chain_like = (int64_t)P->n_levels * 4 > (int64_t)P->n;
     // a supernodal layout the caller built (sptrsv_plan_build_supernodes): the MFMA kernel
     if (!exact && P->sn > 0 && P->relaxed_state != 1 && precision == SPTRSV_FP64) return SPTRSV_VAR_SUPERNODAL;
-    if (!exact && chain_like && precision == SPTRSV_FP64 && relaxed_supernodes_pay(P)) return SPTRSV_VAR_SUPERNODAL;
+    if (!exact && precision == SPTRSV_FP64 && relaxed_supernodes_pay(P)) return SPTRSV_VAR_SUPERNODAL;
     if (!exact && m == 1 && chain_like) return SPTRSV_VAR_CHAIN;
     if (m <= 3 && precision == SPTRSV_FP64 && P->grid && block_supports(P, m)) return SPTRSV_VAR_BLOCK;
     if (m >= 32 && precision == SPTRSV_FP64 && panel_supports(P)) return SPTRSV_VAR_PANEL;

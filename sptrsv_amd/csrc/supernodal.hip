@@ -920,8 +920,9 @@ int download_csr(sptrsv_plan_s* P, std::vector<int>& rp, std::vector<int>& col, 
     return SPTRSV_OK;
 }
 
-// AUTO on a chain-like factor: relaxed supernodes of at least 32 rows on average turn its
-// dependency chain of n rows into one of n / w blocks, each solved on FP64 MFMA.  Built once.
+// AUTO on a factor of 16 or more entries per row: relaxed supernodes of at least 32 rows on
+// average turn its dependency chain of n rows into one of n / w blocks, each solved on FP64
+// MFMA (and its level sets into far fewer block levels).  Built once.
 bool relaxed_supernodes_pay(sptrsv_plan_s* P) {
     if (P->relaxed_state == 0) {
         P->relaxed_state = -1;

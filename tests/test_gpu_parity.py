@@ -332,7 +332,8 @@ def test_ccs2ccs_pad_device_rejects_bad_partition(plans):
 def test_auto_variant(plans):
     """AUTO sends near-chain factors (levels hold < 4 rows on average) to the chain kernel at
     m == 1, natural-order grid factors to the block kernel at m <= 3 in FP64, a plan with a
-    supernodal layout to the supernodal kernel in FP64, and everything else to the
+    supernodal layout (the caller's, or relaxed supernodes of a factor dense enough) to the
+    supernodal kernel in FP64, and everything else to the
     level-ordered pull kernel (block and pull bitwise exact)."""
     assert plans["ic0_12"].auto_variant(1) == capi.VAR_BLOCK
     assert plans["ic0_12"].auto_variant(2) == capi.VAR_BLOCK
@@ -355,6 +356,16 @@ def test_auto_variant(plans):
         assert PS.auto_variant(8, capi.FP32) == capi.VAR_LEVEL_PULL
         xt, b = gen.rhs_for(S, 8, seed=3, layout="row")
         assert O.rel_error(oracle_solve(S, b), PS.solve_host(b, capi.params())) <= 1e-12
+    # a supernodal factor with many rows per level and no layout from the caller: relaxed
+    # supernodes (>= 16 entries per row, >= 32 rows per supernode on average), built by AUTO
+    S = gen.random_supernodal(64, seed=32, wmin=32, wmax=96)
+    with capi.Plan.from_upper(S) as PS:
+        assert PS.n_levels * 4 <= S.n and S.nnz >= 16 * S.n
+        assert PS.auto_variant(8) == capi.VAR_SUPERNODAL
+        assert PS.auto_variant(8, capi.FP32) == capi.VAR_LEVEL_PULL
+        for m in (1, 8):
+            xt, b = gen.rhs_for(S, m, seed=4, layout="row")
+            assert O.rel_error(oracle_solve(S, b), PS.solve_host(b, capi.params())) <= 1e-12
     for name in MATS:
         U, P = MATS[name], plans[name]
         for m in (1, 2, 3, 4):

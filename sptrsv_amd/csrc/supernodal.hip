@@ -920,9 +920,10 @@ int download_csr(sptrsv_plan_s* P, std::vector<int>& rp, std::vector<int>& col, 
     return SPTRSV_OK;
 }
 
-// AUTO on a factor of 16 or more entries per row: relaxed supernodes of at least 32 rows on
-// average turn its dependency chain of n rows into one of n / w blocks, each solved on FP64
-// MFMA (and its level sets into far fewer block levels).  Built once.
+// AUTO on a factor of 16 or more entries per row: supernodes of at least 32 rows on average
+// (without fill if the factor has such, else relaxed) turn its dependency chain of n rows into
+// one of n / w blocks, each solved on FP64 MFMA (and its level sets into far fewer block
+// levels).  Built once.
 bool relaxed_supernodes_pay(sptrsv_plan_s* P) {
     if (P->relaxed_state == 0) {
         P->relaxed_state = -1;
@@ -930,10 +931,12 @@ bool relaxed_supernodes_pay(sptrsv_plan_s* P) {
             std::vector<int> rp, col;
             std::vector<double> val;
             if (download_csr(P, rp, col, val) == SPTRSV_OK) {
-                const std::vector<int> b = relaxed_bounds(P->n, rp, col, kMaxW, 1.0);
-                if ((double)P->n / (double)(b.size() - 1) >= 32.0 &&
-                    build_sn_layout(P, rp, col, val, b, false) == SPTRSV_OK)
-                    P->relaxed_state = 1;
+                // supernodes without fill first (the factor's own, where it has them wide
+                // enough); else relaxed ones that pad up to as many entries as the rows hold
+                const auto wide = [&](const std::vector<int>& b) { return (double)P->n / (double)(b.size() - 1) >= 32.0; };
+                std::vector<int> b = relaxed_bounds(P->n, rp, col, kMaxW, 0.0);
+                if (!wide(b)) b = relaxed_bounds(P->n, rp, col, kMaxW, 1.0);
+                if (wide(b) && build_sn_layout(P, rp, col, val, b, false) == SPTRSV_OK) P->relaxed_state = 1;
             }
         }
     }

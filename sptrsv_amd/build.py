@@ -19,7 +19,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libsptrsv_amd.so")
 ARCH = os.environ.get("SPTRSV_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "analysis.hip", "solve.hip", "supernodal.hip", "tile.hip", "chain.hip", "chain_levels.hip", "batch.hip", "mlp.hip", "block.hip", "panel.hip", "levelset.hip", "select.hip", "refine.hip"]
+HIP_SOURCES = ["runtime.hip", "analysis.hip", "solve.hip", "supernodal.hip", "tile.hip", "chain.hip", "chain_levels.hip", "batch.hip", "mlp.hip", "radix.hip", "block.hip", "panel.hip", "levelset.hip", "select.hip", "refine.hip"]
 CXX_SOURCES = ["dropin.cpp", "gen.cpp"]
 OPENMP_HIP_SOURCES = {"batch.hip"}  # host-side loops over systems run in parallel
 

@@ -5,12 +5,11 @@
 //   level sets + level pointers (rows ascending per level) src/core.cpp:282-287
 //   level-ordered CSR (rows stored level by level)         (no counterpart: feeds the
 //                                                           level-scheduled sync-free kernel)
-// All integer results are exact; the tests compare them bit for bit with the oracle.
+// All integer results are exact; the tests compare them bit for bit with the oracle.  Sorting
+// and prefix sums are this library's own (radix.hip).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
-
-#include <hipcub/hipcub.hpp>
 #include <string>
 #include <vector>
 
@@ -38,10 +37,11 @@ __global__ void k_expand_rows(const int* __restrict__ rowptr, int n, int* __rest
         for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) row_of[p] = r;
 }
 
-// histogram: counts[key[k] + shift]++
-__global__ void k_histogram(const int* __restrict__ keys, int64_t count, int shift, int* __restrict__ counts) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < count; i += (int64_t)gridDim.x * blockDim.x) atomicAdd(&counts[keys[i] + shift], 1);
+// entries of every row: the in-degree the reference's analyser counts as a histogram over the
+// CSC row indices (every entry of row r, the diagonal included, has row index r)
+__global__ void k_row_lengths(const int* __restrict__ rowptr, int n, int* __restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    for (; r < n; r += gridDim.x * blockDim.x) out[r] = rowptr[r + 1] - rowptr[r];
 }
 
 __global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict__ row_of,
@@ -148,18 +148,19 @@ __global__ void k_level_rowlen(const int* __restrict__ rowptr, const int* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) lens[0] = 0;
 }
 
-// copy row level_rows[k] into slot k of the level-ordered CSR; one wave per row
+// copy row level_rows[k] into slot k of the level-ordered CSR; G lanes per row (G a power of two
+// dividing the wave, about the mean row length)
 __global__ void k_level_gather(const int* __restrict__ rowptr, const int* __restrict__ col,
                                const double* __restrict__ val, const int* __restrict__ level_rows,
-                               const int* __restrict__ lrowptr, int n, int* __restrict__ lcol,
+                               const int* __restrict__ lrowptr, int n, int G, int* __restrict__ lcol,
                                double* __restrict__ lval) {
-    const int lane = threadIdx.x & (kWave - 1);
-    int k = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int stride = gridDim.x * blockDim.x / kWave;
-    for (; k < n; k += stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = (int)(t & (G - 1));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x / G;
+    for (int64_t k = t / G; k < n; k += stride) {
         const int r = level_rows[k];
         const int a = rowptr[r], len = rowptr[r + 1] - a, d = lrowptr[k];
-        for (int e = lane; e < len; e += kWave) {
+        for (int e = sub; e < len; e += G) {
             lcol[d + e] = col[a + e];
             lval[d + e] = val[a + e];
         }
@@ -185,38 +186,41 @@ struct DevBuf {
     }
 };
 
+size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int key_bits(int64_t kmax) {  // bits of the largest key
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) <= kmax) ++b;
+    return b;
+}
+
 }  // namespace
 
-// Stable CSR -> CSC on the device: radix-sort the CSR positions by column (stable, so rows
-// stay ascending inside every column -- the reference's counting-sort order), then gather.
+size_t transpose_scratch_bytes(int64_t nnz) {
+    return 3 * align_up((size_t)std::max<int64_t>(nnz, 1) * sizeof(int)) + sort_scratch_bytes(nnz);
+}
+
+// Stable CSR -> CSC on the device: sort the CSR positions by column (stable, so rows stay
+// ascending inside every column -- the reference's counting-sort order), gather rows and values,
+// and take colptr from the sorted columns.  scratch: transpose_scratch_bytes(nnz) bytes, or
+// nullptr to allocate them here.
 int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, const double* val,
-                     int* colptr, int* rowidx, double* cval, hipStream_t s) {
-    DevBuf row_of, pos_in, pos_out, keys_out, tmp;
-    SPT_HIP(hipMalloc(&row_of.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
-    SPT_HIP(hipMalloc(&pos_in.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
-    SPT_HIP(hipMalloc(&pos_out.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
-    SPT_HIP(hipMalloc(&keys_out.p, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
-    k_expand_rows<<<grid_for(n), 256, 0, s>>>(rowptr, n, (int*)row_of.p);
-    k_iota<<<grid_for(nnz), 256, 0, s>>>((int*)pos_in.p, nnz);
-    int end_bit = 1;
-    while (end_bit < 31 && (1 << end_bit) <= n) ++end_bit;
-    size_t tmp_bytes = 0, need = 0;
-    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, col, (int*)keys_out.p, (int*)pos_in.p,
-                                               (int*)pos_out.p, (int)nnz, 0, end_bit, s));
-    tmp_bytes = need;
-    SPT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, colptr, colptr, n + 1, s));
-    tmp_bytes = std::max(tmp_bytes, need);
-    SPT_HIP(hipMalloc(&tmp.p, tmp_bytes));
-    need = tmp_bytes;
-    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, col, (int*)keys_out.p, (int*)pos_in.p,
-                                               (int*)pos_out.p, (int)nnz, 0, end_bit, s));
-    k_gather_csc<<<grid_for(nnz), 256, 0, s>>>((int*)pos_out.p, (int*)row_of.p, val, nnz, rowidx, cval);
-    // colptr: histogram of columns (shifted by one) then inclusive scan
-    k_fill_int<<<grid_for(n + 1), 256, 0, s>>>(colptr, 0, n + 1);
-    k_histogram<<<grid_for(nnz), 256, 0, s>>>(col, nnz, 1, colptr);
-    need = tmp_bytes;
-    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, colptr, colptr, n + 1, s));
-    SPT_HIP(hipStreamSynchronize(s));
+                     int* colptr, int* rowidx, double* cval, hipStream_t s, void* scratch) {
+    DevBuf own;
+    if (!scratch) {
+        SPT_HIP(hipMalloc(&own.p, transpose_scratch_bytes(nnz)));
+        scratch = own.p;
+    }
+    const size_t z = align_up((size_t)std::max<int64_t>(nnz, 1) * sizeof(int));
+    char* at = static_cast<char*>(scratch);
+    int* row_of = reinterpret_cast<int*>(at);
+    int* col_sorted = reinterpret_cast<int*>(at + z);
+    int* pos_sorted = reinterpret_cast<int*>(at + 2 * z);
+    k_expand_rows<<<grid_for(n), 256, 0, s>>>(rowptr, n, row_of);
+    SPT_CHECK(sort_pairs(col, nullptr, nnz, key_bits(n), col_sorted, pos_sorted, at + 3 * z, s));
+    if (nnz > 0) k_gather_csc<<<grid_for(nnz), 256, 0, s>>>(pos_sorted, row_of, val, nnz, rowidx, cval);
+    SPT_CHECK(key_bounds(col_sorted, nnz, n, 0, colptr, s));
+    if (own.p) SPT_HIP(hipStreamSynchronize(s));
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }
@@ -226,21 +230,15 @@ int analyse(sptrsv_plan_s* p) {
     const int64_t nnz = p->nnz;
     hipStream_t s = p->stream;
 
-    DevBuf tmp, keys_out, pos_in;
-    size_t tmp_bytes = 0, need = 0;
-    SPT_CHECK(device_transpose(n, nnz, p->rowptr, p->col, p->val, p->colptr, p->rowidx, p->cval, s));
-    SPT_HIP(hipMalloc(&keys_out.p, (size_t)n * sizeof(int)));
-    SPT_HIP(hipMalloc(&pos_in.p, (size_t)n * sizeof(int)));
-    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, p->levels, (int*)keys_out.p,
-                                               (int*)pos_in.p, p->level_rows, n, 0, 31, s));
-    tmp_bytes = need;
-    SPT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, p->lrowptr, p->lrowptr, n + 1, s));
-    tmp_bytes = std::max(tmp_bytes, need);
-    SPT_HIP(hipMalloc(&tmp.p, tmp_bytes));
+    // one work buffer for the transpose and then the level sort
+    const size_t zn = align_up((size_t)std::max(n, 1) * sizeof(int));
+    const size_t level_bytes = zn + sort_scratch_bytes(n) + scan_scratch_bytes((int64_t)n + 1);
+    DevBuf work;
+    SPT_HIP(hipMalloc(&work.p, std::max(transpose_scratch_bytes(nnz), level_bytes)));
+    SPT_CHECK(device_transpose(n, nnz, p->rowptr, p->col, p->val, p->colptr, p->rowidx, p->cval, s, work.p));
 
-    // ---- in-degree: the reference analyser's histogram over CSC row indices ---------------
-    k_fill_int<<<grid_for(n), 256, 0, s>>>(p->indeg, 0, n);
-    k_histogram<<<grid_for(nnz), 256, 0, s>>>(p->rowidx, nnz, 0, p->indeg);
+    // ---- in-degree: entries per row ------------------------------------------------------
+    k_row_lengths<<<grid_for(n), 256, 0, s>>>(p->rowptr, n, p->indeg);
 
     // ---- levels --------------------------------------------------------------------------
     k_fill_int<<<grid_for(n), 256, 0, s>>>(p->levels, 0, n);
@@ -255,8 +253,9 @@ int analyse(sptrsv_plan_s* p) {
     } else {
         const int waves = (n + kWave - 1) / kWave;
         const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
-        // persistent waves: as many as the device holds at once, or fewer for small factors
-        const int blocks = std::min((waves + 3) / 4, device_compute_units() * 8);
+        // persistent waves, 8 per CU (32 were slower: the polls of rows in flight contend), or
+        // fewer for small factors
+        const int blocks = std::min((waves + 3) / 4, device_compute_units() * 2);
         k_levels<<<blocks, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
     }
     int ctrl[3] = {0, 0, 0};
@@ -268,27 +267,21 @@ int analyse(sptrsv_plan_s* p) {
     const int L = ctrl[2];
     p->n_levels = L;
 
-    // ---- level pointers: histogram of levels + inclusive scan (level_ptr[0] = 0) -------------
+    // ---- level sets: row ids stably sorted by level (rows ascending per level), and the
+    // level pointers from the sorted levels (level_ptr[0] = 0) ------------------------------
+    int* lev_sorted = static_cast<int*>(work.p);
+    char* rest = static_cast<char*>(work.p) + zn;
     SPT_HIP(hipMalloc((void**)&p->level_ptr, ((size_t)L + 1) * sizeof(int)));
-    k_fill_int<<<grid_for(L + 1), 256, 0, s>>>(p->level_ptr, 0, L + 1);
-    k_histogram<<<grid_for(n), 256, 0, s>>>(p->levels, n, 0, p->level_ptr);
-    need = tmp_bytes;
-    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, p->level_ptr, p->level_ptr, L + 1, s));
-
-    // ---- level sets: stable sort of row ids by level (rows stay ascending per level) ------
-    int lbits = 1;
-    while (lbits < 31 && (1 << lbits) <= L) ++lbits;
-    k_iota<<<grid_for(n), 256, 0, s>>>((int*)pos_in.p, n);
-    need = tmp_bytes;
-    SPT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, p->levels, (int*)keys_out.p,
-                                               (int*)pos_in.p, p->level_rows, n, 0, lbits, s));
+    SPT_CHECK(sort_pairs(p->levels, nullptr, n, key_bits(L), lev_sorted, p->level_rows, rest, s));
+    SPT_CHECK(key_bounds(lev_sorted, n, L, 1, p->level_ptr, s));
 
     // ---- level-ordered CSR ---------------------------------------------------------------
     k_level_rowlen<<<grid_for(n), 256, 0, s>>>(p->rowptr, p->level_rows, n, p->lrowptr);
-    need = tmp_bytes;
-    SPT_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, p->lrowptr, p->lrowptr, n + 1, s));
-    k_level_gather<<<grid_for((int64_t)n * kWave), 256, 0, s>>>(p->rowptr, p->col, p->val, p->level_rows,
-                                                                p->lrowptr, n, p->lcol, p->lval);
+    SPT_CHECK(scan_ints(p->lrowptr, p->lrowptr, (int64_t)n + 1, true, rest, s));
+    int G = 1;
+    while (G < kWave && (int64_t)G * n < nnz) G *= 2;
+    k_level_gather<<<grid_for((int64_t)n * G), 256, 0, s>>>(p->rowptr, p->col, p->val, p->level_rows, p->lrowptr,
+                                                             n, G, p->lcol, p->lval);
     p->h_level_ptr.resize((size_t)L + 1);
     SPT_HIP(hipMemcpyAsync(p->h_level_ptr.data(), p->level_ptr, ((size_t)L + 1) * sizeof(int),
                            hipMemcpyDeviceToHost, s));

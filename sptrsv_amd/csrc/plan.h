@@ -182,7 +182,18 @@ int analyse(sptrsv_plan_s* p);           // analysis.hip
 int chain_levels(sptrsv_plan_s* p, int* ctrl);  // chain_levels.hip: levels of a chain-like factor
 int ensure_fp32(sptrsv_plan_s* p);       // analysis.hip
 int device_transpose(int n, int64_t nnz, const int* rowptr, const int* col, const double* val,
-                     int* colptr, int* rowidx, double* cval, hipStream_t s);  // analysis.hip
+                     int* colptr, int* rowidx, double* cval, hipStream_t s, void* scratch = nullptr);
+size_t transpose_scratch_bytes(int64_t nnz);
+
+// radix.hip: stable radix sort of int (key, value) pairs (keys in [0, 2^bits); vals == nullptr
+// sorts the item indices), prefix sums of ints, and the pointer array of a sorted key list
+// (ptr[c] = first position whose key - base >= c, c in [0, K]).  in/out of scan_ints may alias.
+size_t sort_scratch_bytes(int64_t N);
+int sort_pairs(const int* keys_in, const int* vals_in, int64_t N, int bits, int* keys_out, int* vals_out,
+               void* scratch, hipStream_t s);
+size_t scan_scratch_bytes(int64_t count);
+int scan_ints(const int* in, int* out, int64_t count, bool inclusive, void* scratch, hipStream_t s);
+int key_bounds(const int* sorted, int64_t N, int K, int base, int* ptr, hipStream_t s);  // analysis.hip
 int supernodal_solve(sptrsv_plan_s* p, int precision, const double* d_b, double* d_x, int m,
                      uint64_t limit);  // supernodal.hip
 void supernodal_free(sptrsv_plan_s* p);   // supernodal.hip

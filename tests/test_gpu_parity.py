@@ -99,6 +99,25 @@ def test_device_transpose_general_pattern():
     assert np.array_equal(cp, C.indptr) and np.array_equal(ri, C.indices) and np.array_equal(cv, C.data)
 
 
+@pytest.mark.parametrize("n,per_row", [(70000, 7), (4096, 1), (200, 150), (1 << 17, 0)])
+def test_device_transpose_sort_sizes(n, per_row):
+    """the transpose's own stable radix sort over many tiles and digit passes (n = 70,000: three
+    8-bit passes, ~120 tiles of 4,096 entries, a two-level scan of the digit histogram),
+    one column per row, dense rows, and columns left empty (per_row 0: a few entries only)"""
+    rng = np.random.default_rng(n + per_row)
+    import scipy.sparse as sp
+    k = per_row if per_row else 0
+    rows = np.repeat(np.arange(n), k) if k else rng.integers(0, n, 37)
+    cols = rng.integers(0, n, len(rows))
+    A = sp.csr_matrix((rng.standard_normal(len(rows)), (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    cp, ri, cv = capi.csr_to_csc(n, A.indptr, A.indices, A.data)
+    C = A.tocsc()
+    C.sort_indices()
+    assert np.array_equal(cp, C.indptr) and np.array_equal(ri, C.indices) and np.array_equal(cv, C.data)
+
+
 @pytest.mark.parametrize("variant", EXACT_M1 + [capi.VAR_CSC_PUSH])
 @pytest.mark.parametrize("name", list(MATS))
 def test_fp64_single_rhs(plans, name, variant):

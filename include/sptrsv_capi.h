@@ -185,12 +185,15 @@ int sptrsv_plan_get_csc(sptrsv_plan_t plan, int* colptr /* n+1 */, int* rowidx /
                         double* val /* nnz */);
 
 /* The variant SPTRSV_VAR_AUTO runs for m right-hand sides at `precision` (no reference
- * counterpart).  For a chain-like factor (levels hold < 4 rows on average): SUPERNODAL in FP64
- * when relaxed supernodes pay (see sptrsv_plan_build_supernodes_relaxed; built here on first
- * use), else CHAIN for m == 1.  BLOCK for FP64 m <= 3 on a natural-order grid factor;
- * LEVEL_PULL otherwise.  With params.exact only BLOCK and LEVEL_PULL are chosen.  For 2 <= m <= 3 on a grid
- * factor the choice checks the block layout's limits and so builds that layout (on the host,
- * once per plan; the block solve would build it on its first call). */
+ * counterpart).  With a selector network installed (sptrsv_selector_set), its choice.  Else, in
+ * this order: SUPERNODAL in FP64 on a plan with supernodes -- the caller's
+ * (sptrsv_plan_build_supernodes), or, for a factor of 16 or more entries per row, its own
+ * fill-free supernodes or else relaxed ones when they average 32 rows or more (built here on
+ * first use); CHAIN for m == 1 on a chain-like factor (levels hold < 4 rows on average); BLOCK
+ * for FP64 m <= 3 on a natural-order grid factor; PANEL for FP64 m >= 32 on one; LEVEL_PULL
+ * otherwise.  With params.exact only the bitwise kernels are chosen.  For 2 <= m <= 3 on a
+ * grid factor the choice checks the block layout's limits and so builds that layout (on the
+ * host, once per plan; the block solve would build it on its first call). */
 int sptrsv_plan_auto_variant(sptrsv_plan_t plan, int m, int precision, int* variant);
 /* Learned choice of the variant for AUTO (no reference counterpart).  The 24 features of the
  * plan's factor at m right-hand sides (the batched selector's: log2 n, log2 nnz, row-length

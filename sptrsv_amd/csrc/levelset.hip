@@ -4,8 +4,11 @@
 // The rows of a level are independent; the reference gives each thread a share of a level and
 // puts an OpenMP barrier between levels.  Here every level's rows (consecutive slots of the
 // level-ordered CSR, so a level's entries are one contiguous slice) are cut into chunks of at
-// most kChunkRows rows and kChunkEntries entries, built once per plan, and each chunk is one
-// workgroup.  Workgroups take chunks by ticket in level order; a chunk waits until the
+// most max(16, 256 / G) rows (G lanes per row over the right-hand sides: at most four
+// (row, column) items per thread, so a level costs about one round of dependency loads whatever
+// m is; smaller chunks only add contention on the ticket and level counters: on the 128^3
+// factor at m = 32, 9.4 ms with 16 rows, 10.5 with 4, 25 with 256) and kChunkEntries entries,
+// built once per plan and chunk size, and each chunk is one workgroup.  Workgroups take chunks by ticket in level order; a chunk waits until the
 // previous level's completion counter has counted all of that level's chunks (held by
 // workgroups with smaller tickets, which are running: no co-residency is needed, and no
 // cooperative launch), stages its column indices and values in LDS with coalesced loads,
@@ -28,7 +31,7 @@ namespace sptrsv {
 namespace {
 
 constexpr int kLT = 256;             // threads per workgroup
-constexpr int kChunkRows = 256;
+constexpr int kMinChunkRows = 16;    // fewer rows per chunk: more chunks contend on the counters
 constexpr int kChunkEntries = 3072;  // staged entries: 36 KiB of LDS (int + double each)
 constexpr int kTicketSlot = 32;      // this solve's ticket counter in the plan's control words
 
@@ -118,9 +121,9 @@ void launch_levelset(int G, unsigned grid, const LevelSetArgs<T>& a, hipStream_t
     }
 }
 
-// Chunks of every level: consecutive slots, at most kChunkRows rows and kChunkEntries entries
-// (a longer row is a chunk of its own, read from the global CSR).
-int build_chunks(sptrsv_plan_s* P) {
+// Chunks of every level: consecutive slots, at most `rows` rows and kChunkEntries entries (a
+// longer row is a chunk of its own, read from the global CSR), into chunk set `si`.
+int build_chunks(sptrsv_plan_s* P, int si, int rows) {
     const int n = P->n, L = P->n_levels;
     std::vector<int> lrp((size_t)n + 1);
     SPT_HIP(hipStreamSynchronize(P->stream));
@@ -131,7 +134,7 @@ int build_chunks(sptrsv_plan_s* P) {
         int s = lo;
         while (s < hi) {
             int e = s + 1;
-            while (e < hi && e - s < kChunkRows && lrp[(size_t)e + 1] - lrp[(size_t)s] <= kChunkEntries) ++e;
+            while (e < hi && e - s < rows && lrp[(size_t)e + 1] - lrp[(size_t)s] <= kChunkEntries) ++e;
             ck_slot.push_back(s);
             ck_lev.push_back(l);
             s = e;
@@ -139,43 +142,49 @@ int build_chunks(sptrsv_plan_s* P) {
         ck_level.push_back((int)ck_slot.size());
     }
     ck_slot.push_back(n);
-    levelset_free(P);
     auto up = [](int** dst, const std::vector<int>& v) {
         SPT_HIP(hipMalloc((void**)dst, std::max<size_t>(v.size(), 1) * sizeof(int)));
         if (!v.empty()) SPT_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
         return SPTRSV_OK;
     };
-    SPT_CHECK(up(&P->ls_ck_level, ck_level));
-    SPT_CHECK(up(&P->ls_ck_slot, ck_slot));
-    SPT_CHECK(up(&P->ls_ck_lev, ck_lev));
-    SPT_HIP(hipMalloc((void**)&P->ls_done, std::max<size_t>((size_t)L, 1) * sizeof(int)));
-    P->ls_chunks = (int)ck_lev.size();
+    SPT_CHECK(up(&P->ls_ck_level[si], ck_level));
+    SPT_CHECK(up(&P->ls_ck_slot[si], ck_slot));
+    SPT_CHECK(up(&P->ls_ck_lev[si], ck_lev));
+    if (!P->ls_done) SPT_HIP(hipMalloc((void**)&P->ls_done, std::max<size_t>((size_t)L, 1) * sizeof(int)));
+    P->ls_chunks[si] = (int)ck_lev.size();
     return SPTRSV_OK;
 }
 
 }  // namespace
 
 void levelset_free(sptrsv_plan_s* p) {
-    for (int* q : {p->ls_ck_level, p->ls_ck_slot, p->ls_ck_lev, p->ls_done})
-        if (q) (void)hipFree(q);
-    p->ls_ck_level = p->ls_ck_slot = p->ls_ck_lev = p->ls_done = nullptr;
-    p->ls_chunks = 0;
+    for (int si = 0; si < sptrsv_plan_s::kLsSizes; ++si) {
+        for (int* q : {p->ls_ck_level[si], p->ls_ck_slot[si], p->ls_ck_lev[si]})
+            if (q) (void)hipFree(q);
+        p->ls_ck_level[si] = p->ls_ck_slot[si] = p->ls_ck_lev[si] = nullptr;
+        p->ls_chunks[si] = 0;
+    }
+    if (p->ls_done) (void)hipFree(p->ls_done);
+    p->ls_done = nullptr;
 }
 
 template <typename T>
 int levelset_solve(sptrsv_plan_s* P, const T* lval, const T* d_b, T* d_x, int m, uint64_t limit) {
     if (P->n == 0) return SPTRSV_OK;
-    if (!P->ls_ck_level) SPT_CHECK(build_chunks(P));
-    int G = 1;
-    while (G < 64 && G < m) G *= 2;
+    int G = 1, si = 0;
+    while (G < 64 && G < m) {
+        G *= 2;
+        ++si;
+    }
+    if (!P->ls_ck_level[si]) SPT_CHECK(build_chunks(P, si, std::max(kMinChunkRows, kLT / G)));
     // the level counters and the ticket start at zero, in stream order before the solve
     SPT_CHECK(zero_ints_async(P->ls_done, P->n_levels, P->stream));
     SPT_CHECK(zero_ints_async(P->ticket + kTicketSlot, 1, P->stream));
-    LevelSetArgs<T> a{m, P->ls_ck_level, P->ls_ck_slot, P->ls_ck_lev, P->lrowptr, P->lcol, lval, P->level_rows,
-                      d_b, d_x, P->ls_done, P->ticket + kTicketSlot, P->ticket + 1, limit};
+    LevelSetArgs<T> a{m, P->ls_ck_level[si], P->ls_ck_slot[si], P->ls_ck_lev[si], P->lrowptr, P->lcol, lval,
+                      P->level_rows, d_b, d_x, P->ls_done, P->ticket + kTicketSlot, P->ticket + 1, limit};
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
-    launch_levelset<T>(G, (unsigned)P->ls_chunks, a, P->stream);
+    launch_levelset<T>(G, (unsigned)P->ls_chunks[si], a, P->stream);
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;

@@ -146,15 +146,17 @@ struct sptrsv_plan_s {
     int pn_epoch = 0;
     unsigned pn_tickets = 0;
 
-    // chunks of the level-set solve (levelset.hip, built on first use): chunks of level l are
-    // ls_ck_level[l] .. ls_ck_level[l+1]; chunk c is level-ordered slots
-    // [ls_ck_slot[c], ls_ck_slot[c+1]) of level ls_ck_lev[c]; ls_done: per level, chunks
-    // finished in the current solve
-    int* ls_ck_level = nullptr;
-    int* ls_ck_slot = nullptr;
-    int* ls_ck_lev = nullptr;
+    // chunks of the level-set solve (levelset.hip, built on first use), one set per chunk size
+    // (256 >> s rows for G = 1 << s lanes per row): chunks of level l are
+    // ls_ck_level[s][l] .. ls_ck_level[s][l+1]; chunk c is level-ordered slots
+    // [ls_ck_slot[s][c], ls_ck_slot[s][c+1]) of level ls_ck_lev[s][c]; ls_done: per level,
+    // chunks finished in the current solve
+    static constexpr int kLsSizes = 7;
+    int* ls_ck_level[kLsSizes] = {};
+    int* ls_ck_slot[kLsSizes] = {};
+    int* ls_ck_lev[kLsSizes] = {};
+    int ls_chunks[kLsSizes] = {};
     int* ls_done = nullptr;
-    int ls_chunks = 0;
 
     // chain variant: one record per 32-row window (packed inverse of its triangle, near and old
     // entries), built on first use; FP64 and FP32 values

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -29,8 +30,7 @@ struct Net {
     std::vector<int> classes;  // output index -> SPTRSV_VAR_*
 };
 std::mutex g_net_mutex;
-Net g_net;
-bool g_net_set = false;
+std::shared_ptr<const Net> g_net;  // null: no network installed
 
 // The factor's 24 features (feature 22, log2 m, filled per call); computed once per plan from
 // its CSR and levels.
@@ -78,12 +78,13 @@ bool variant_valid(sptrsv_plan_s* P, int v, int m, int precision, bool exact) {
 // AUTO's choice by the installed network; false when none is installed or none of the variants
 // it scores is valid here.
 bool selector_pick(sptrsv_plan_s* P, int m, int precision, bool exact, int* variant) {
-    Net net;
+    std::shared_ptr<const Net> held;  // the network stays alive while it is read, even if replaced
     {
         std::lock_guard<std::mutex> lk(g_net_mutex);
-        if (!g_net_set) return false;
-        net = g_net;
+        held = g_net;
     }
+    if (!held) return false;
+    const Net& net = *held;
     float x[24];
     if (net.in != 24 || plan_features_cached(P, m, x) != SPTRSV_OK) return false;
     for (int k = 0; k < 24; ++k) x[k] = (x[k] - net.mu[(size_t)k]) / net.sd[(size_t)k];
@@ -139,26 +140,27 @@ int sptrsv_selector_set(int in, int h1, int h2, int out, const float* params, co
         return fail(SPTRSV_ERR_BAD_INPUT, "selector: sizes 24 x h1 x h2 x out and all arrays required");
     for (int k = 0; k < out; ++k)
         if (classes[k] < 1 || classes[k] >= SPTRSV_VAR_COUNT) return fail(SPTRSV_ERR_BAD_INPUT, "selector: bad variant id");
-    Net net;
-    net.in = in;
-    net.h1 = h1;
-    net.h2 = h2;
-    net.out = out;
+    for (int k = 0; k < in; ++k)
+        if (!(sd[k] > 0.0f) || !std::isfinite(sd[k]) || !std::isfinite(mu[k]))
+            return fail(SPTRSV_ERR_BAD_INPUT, "selector: feature scales must be finite, sd > 0");
+    auto net = std::make_shared<Net>();
+    net->in = in;
+    net->h1 = h1;
+    net->h2 = h2;
+    net->out = out;
     const size_t np = (size_t)in * h1 + h1 + (size_t)h1 * h2 + h2 + (size_t)h2 * out + out;
-    net.params.assign(params, params + np);
-    net.mu.assign(mu, mu + in);
-    net.sd.assign(sd, sd + in);
-    net.classes.assign(classes, classes + out);
+    net->params.assign(params, params + np);
+    net->mu.assign(mu, mu + in);
+    net->sd.assign(sd, sd + in);
+    net->classes.assign(classes, classes + out);
     std::lock_guard<std::mutex> lk(g_net_mutex);
     g_net = std::move(net);
-    g_net_set = true;
     return SPTRSV_OK;
 }
 
 int sptrsv_selector_clear(void) {
     std::lock_guard<std::mutex> lk(g_net_mutex);
-    g_net_set = false;
-    g_net = Net();
+    g_net.reset();
     return SPTRSV_OK;
 }
 

@@ -51,6 +51,23 @@ def test_selector_beats_fallbacks_on_heldout_factors():
     assert r_push > 10 * max(r_mlp, 0.01)
 
 
+def test_selector_set_validates_its_inputs():
+    """host-only entry points (no device call): a network with a bad variant id, a zero or
+    non-finite feature scale, or wrong input width is rejected; a valid one installs and clears"""
+    doc, *_ = load()
+    sizes, params, mu, sd, classes = doc["sizes"], doc["params"], doc["mu"], doc["sd"], doc["classes"]
+    bad_sd = list(sd)
+    bad_sd[3] = 0.0
+    nan_mu = list(mu)
+    nan_mu[0] = float("nan")
+    for args in ((sizes, params, mu, sd, [0] + classes[1:]), (sizes, params, mu, bad_sd, classes),
+                 (sizes, params, nan_mu, sd, classes), ([23] + sizes[1:], params, mu, sd, classes)):
+        with pytest.raises(capi.SptrsvError):
+            capi.selector_set(*args)
+    capi.selector_set(sizes, params, mu, sd, classes)
+    capi.selector_clear()
+
+
 @pytest.mark.gpu
 def test_library_forward_and_auto_match_numpy():
     """the library's forward pass on a plan's features picks the variant numpy's forward pass

@@ -82,6 +82,8 @@ struct sptrsv_plan_s {
     int sn_epoch = 0;
     std::vector<int> h_snodes;
     std::vector<int> h_sn_r;        // rectangular column count per supernode
+    int sn_rmax = 0;                // the largest of them
+    int sn_levels = 0;              // dependency levels of the supernodes
     int64_t* sn_offr16 = nullptr;   // offsets of the BF16 blocks (rows padded to 16 columns)
     int64_t sn_padded = 0;          // explicit zeros stored by the layout (relaxed supernodes)
     int relaxed_state = 0;          // AUTO's relaxed supernodes: 0 not tried, 1 built, -1 not used

@@ -355,6 +355,115 @@ __global__ void k_reset_ticket_sn(int* ctrl) {
     if (threadIdx.x == 0) ctrl[0] = 0;
 }
 
+// Wave 0 of a workgroup waits (relaxed polls, watchdog) until the flags of the supernodes owning
+// the rectangular columns of supernode k, column panel `panel`, hold this solve's epoch.  The
+// caller follows with a workgroup barrier and an acquire fence.
+__device__ __forceinline__ void sn_wait_deps(const SnArgs& a, int k, int panel, int lane) {
+    Watchdog wd;
+    wd.arm(&a.ctrl[1], a.limit);
+    const int d1 = a.dep_ptr[k + 1];
+    int idle = 0;
+    bool give_up = false;
+    for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
+        const int qd = base + lane;
+        bool ready = qd >= d1;
+        const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[qd] * a.npanel + panel];
+        while (!__all(ready)) {
+            if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+            if (__all(ready)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && __any(wd.expired())) {
+                give_up = true;
+                break;
+            }
+        }
+    }
+}
+
+// One right-hand side.  The MFMA kernel's 16-column tiles would leave 15 of 16 columns idle, and
+// it loads R and Tinv after its wait, one memory latency per chunk; on a chain of supernodes
+// (a banded factor in blocks of 128 rows) those latencies add up along the whole chain.  Here a
+// workgroup of 512 threads per supernode gives each row four lanes (lane q of row i = t / 4
+// takes columns q, q + 4, q + 8, ...) and loads everything that does not depend on X before it
+// waits: b_s, the 4 * kSgPre or fewer columns of R_s with their column indices, and row i of Tinv_s
+// (columns j >= i).  After the wait it gathers X at those columns, forms
+//     W_i   = b_i - R_s(i, :) X(rect)      four partial sums, reduced across the quad,
+//     X_s,i = Tinv_s(i, :) W               W passed through LDS, reduced the same way,
+// and publishes X_s with the flag protocol of k_sn_solve (panel 0).  Every rectangular column
+// is loaded before the wait, so the kernel needs r <= 4 * kSgPre for every supernode (a band of
+// width <= 128 in blocks of 128 rows); the solve takes the MFMA kernel otherwise.  The sums run
+// in a fixed order: the same inputs give the same X on every run.
+constexpr int kSgPre = 32;  // R columns per lane loaded before the wait
+constexpr int kSgT = kMaxW / 4;  // Tinv columns per lane
+
+__global__ __launch_bounds__(512, 1) void k_sn_gemv(SnArgs a) {
+    __shared__ double sW[kMaxW];
+    __shared__ int s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
+    __syncthreads();
+    const int k = a.sn - 1 - s_ticket;
+    if (k < 0) return;
+    const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
+    const int i = tid >> 2, q = tid & 3;
+    const bool row = i < w;
+    const double* R = a.rect + a.offr[k] + (size_t)(row ? i : 0) * r;
+    const int* C = a.cols + a.offc[k];
+    const double* Ti = a.tinv + a.offt[k] + (size_t)(row ? i : 0) * w;
+
+    // ---- independent of X: b, R (first columns), column indices, Tinv row ---------------
+    const double bi = (row && q == 0) ? a.b[s0 + i] : 0.0;
+    double rv[kSgPre], tv[kSgT];
+    int cv[kSgPre];
+#pragma unroll
+    for (int u = 0; u < kSgPre; ++u) {
+        const int c = q + 4 * u;
+        const bool ok = row && c < r;
+        rv[u] = ok ? R[c] : 0.0;
+        cv[u] = ok ? C[c] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kSgT; ++u) {
+        const int j = q + 4 * u;
+        tv[u] = (row && j >= i && j < w) ? Ti[j] : 0.0;
+    }
+
+    // ---- wait for the supernodes owning the rectangular columns --------------------------
+    if (wv == 0) sn_wait_deps(a, k, 0, lane);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // ---- W = b - R X(rect) -----------------------------------------------------------------
+    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < kSgPre; u += 2) {
+        const double x0 = cv[u] >= 0 ? a.x[cv[u]] : 0.0;
+        const double x1 = cv[u + 1] >= 0 ? a.x[cv[u + 1]] : 0.0;
+        acc0 = fma(rv[u], x0, acc0);
+        acc1 = fma(rv[u + 1], x1, acc1);
+    }
+    double s = acc0 + acc1;
+    s += __shfl_xor(s, 1, 4);
+    s += __shfl_xor(s, 2, 4);
+    if (q == 0) sW[i] = row ? bi - s : 0.0;  // zeros past w: Tinv's padding multiplies them
+    __syncthreads();
+
+    // ---- X_s = Tinv_s W ----------------------------------------------------------------------
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < kSgT; u += 2) {
+        t0 = fma(tv[u], sW[q + 4 * u], t0);
+        t1 = fma(tv[u + 1], sW[q + 4 * u + 4], t1);
+    }
+    double xs = t0 + t1;
+    xs += __shfl_xor(xs, 1, 4);
+    xs += __shfl_xor(xs, 2, 4);
+    if (row && q == 0) a.x[s0 + i] = xs;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename X>
 int upload(X** dst, const std::vector<X>& src) {
     SPT_HIP(hipMalloc((void**)dst, std::max<size_t>(src.size(), 1) * sizeof(X)));
@@ -668,6 +777,8 @@ void supernodal_free(sptrsv_plan_s* p) {
     if (p->sn_offr16) (void)hipFree(p->sn_offr16);
     p->sn_offr16 = nullptr;
     p->h_sn_r.clear();
+    p->sn_rmax = 0;
+    p->sn_levels = 0;
     p->sn = 0;
     p->h_snodes.clear();
 }
@@ -682,8 +793,13 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     if (precision != SPTRSV_FP64)
         return fail(SPTRSV_ERR_UNSUPPORTED, "supernodal variant: FP64 only through sptrsv_solve");
     const int nblk = (m + 15) / 16;
+    // One column on a factor whose supernode dependency levels hold at most 256 supernodes on
+    // average (chain-like: the one-workgroup-per-CU residency of k_sn_gemv costs nothing there)
+    // and whose rectangular blocks fit its prefetch: k_sn_gemv.  Else k_sn_solve, NW waves per
+    // panel of NW 16-column blocks; its one-wave form keeps four supernodes per CU resident.
+    const bool gemv = m == 1 && P->sn_rmax <= 4 * kSgPre && P->sn <= 256 * (int64_t)P->sn_levels;
     const int nw = sn_waves(nblk);
-    const int npanel = (nblk + nw - 1) / nw;
+    const int npanel = gemv ? 1 : (nblk + nw - 1) / nw;
     SPT_CHECK(ensure_sn_flags(P, npanel));
     P->sn_epoch++;
     SnArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
@@ -693,7 +809,8 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
     const unsigned grid = (unsigned)((int64_t)P->sn * npanel);
-    if (nw == 8) k_sn_solve<8, false><<<grid, 512, 0, P->stream>>>(a);
+    if (gemv) k_sn_gemv<<<grid, 512, 0, P->stream>>>(a);
+    else if (nw == 8) k_sn_solve<8, false><<<grid, 512, 0, P->stream>>>(a);
     else if (nw == 4) k_sn_solve<4, false><<<grid, 256, 0, P->stream>>>(a);
     else if (nw == 2) k_sn_solve<2, false><<<grid, 128, 0, P->stream>>>(a);
     else k_sn_solve<1, false><<<grid, 64, 0, P->stream>>>(a);
@@ -899,6 +1016,16 @@ int build_sn_layout(sptrsv_plan_s* P, const std::vector<int>& rp, const std::vec
     if (e != hipSuccess) return fail(SPTRSV_ERR_HIP, std::string("supernode inversion: ") + hipGetErrorString(e));
     P->sn = S;
     P->h_sn_r = rcount;
+    P->sn_rmax = rcount.empty() ? 0 : *std::max_element(rcount.begin(), rcount.end());
+    {  // dependency levels of the supernodes (a supernode depends on later ones only)
+        std::vector<int> lv(S, 1);
+        int nl = S > 0 ? 1 : 0;
+        for (int k = S - 1; k >= 0; --k) {
+            for (int q = dep_ptr[k]; q < dep_ptr[k + 1]; ++q) lv[k] = std::max(lv[k], lv[dep[q]] + 1);
+            nl = std::max(nl, lv[k]);
+        }
+        P->sn_levels = nl;
+    }
     P->sn_wmax = wmax;
     P->sn_epoch = 0;
     P->sn_flag_blocks = 1;

@@ -611,9 +611,10 @@ def test_supernodal_on_general_pattern_and_fixture(plans, csr6, snodes6):
     for name in ("zipf_3000", "chain_4000", "chol2d_15"):
         U, P = MATS[name], plans[name]
         P.build_supernodes()
-        xt, b = gen.rhs_for(U, 4, seed=4, layout="row")
-        x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
-        assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, name
+        for m in (1, 4):  # m = 1: the one-column kernel
+            xt, b = gen.rhs_for(U, m, seed=4, layout="row")
+            x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+            assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, (name, m)
     n, rp, col, val = csr6
     with capi.Plan.from_csr(rp, col, val) as P:
         P.build_supernodes(snodes6)
@@ -636,7 +637,7 @@ def test_supernodal_full_size_4096():
     banner(f"plan supernodal_4096 n={U.n} nnz={U.nnz}")
     with capi.Plan.from_upper(U) as P:
         P.build_supernodes(U.snodes)
-        for m in (8,):
+        for m in (1, 8):  # m = 1: the one-column kernel, rectangular blocks up to 6 x 128 columns
             xt, b = gen.rhs_for(U, m, seed=2, layout="row")
             banner(f"solve supernodal_4096 m={m}")
             x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))

@@ -18,9 +18,11 @@
 //      lanes per cell, four columns per lane; dependencies come from the previous step's LDS
 //      buffer or from the halo, b from the lane's registers (overwritten by x); the steps touch
 //      LDS and registers only, so no step waits on memory; an LDS-only barrier separates them;
-//   4. the brick's x leaves in one burst of device-scope stores; once every wave's stores are
-//      performed (vmcnt 0) one lane stores the (brick, panel) flag = this solve's epoch.  Flags
-//      carry the epoch, so nothing is reset between solves and x needs no sentinel fill.
+//   4. the brick's x leaves in one burst of stores, device-scope for the face cells that a
+//      downstream brick reads; once every wave's stores are performed (vmcnt 0) one lane stores
+//      the (brick, panel) flag = this solve's epoch (with at most four panels, the face cells are
+//      stored and flagged first and the other cells after the flag).  Flags carry the epoch,
+//      so nothing is reset between solves and x needs no sentinel fill.
 // Each cell sums its products in ascending column order (+1, +nx, +nx*ny), only over the
 // entries the row has, and divides by the diagonal: the reference serial solver's arithmetic
 // (src/core.cpp:18-32), so x is bitwise equal to it.
@@ -41,6 +43,7 @@ namespace {
 constexpr int kPanel = 16;  // right-hand-side columns per work item
 constexpr int kPT = 256;    // threads: 64 cells per step x 4 lanes, 4 columns per lane
 constexpr int kTicketSlot = 16;  // this kernel's ticket counter in the plan's control words
+constexpr int kSplitPanels = 4;  // face stores flagged first up to this many panels (m <= 64)
 
 struct PanelArgs {
     int nx, ny, nz;
@@ -114,7 +117,7 @@ __device__ __forceinline__ double quad_get(double v) {
 
 // VEC: m a multiple of 4 and b 16-byte aligned, so every lane's 4 columns are all valid or
 // all beyond m, and 16-byte aligned: two 16-byte loads per cell instead of four 8-byte ones.  At most 128 VGPRs: four workgroups per CU.
-template <int BX, int BY, int BZ, bool VEC>
+template <int BX, int BY, int BZ, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kPT, 4) void k_panel(PanelArgs a) {
 #pragma clang fp contract(off)
     using B = Brick<BX, BY, BZ>;
@@ -301,11 +304,15 @@ __global__ __launch_bounds__(kPT, 4) void k_panel(PanelArgs a) {
     });
 
     // 4. x of the brick, through LDS so that sixteen lanes hold one cell's sixteen columns and
-    //    every store instruction writes four whole 128-byte rows: cells that a downstream brick
-    //    reads (local coordinate 0 in a direction with a downstream brick) with device-scope
-    //    stores, the rest with ordinary ones (read only after the kernel); then the flag, once
-    //    every wave's stores are performed
-    {
+    //    every store instruction writes four whole 128-byte rows: the cells that a downstream
+    //    brick reads (local coordinate 0 in a direction with a downstream brick) with
+    //    device-scope stores, the others with ordinary ones (read only after the kernel); the
+    //    flag once every wave's stores are performed.  With SPLIT the face cells go first and
+    //    the flag follows their stores alone; the other cells are staged through LDS again and
+    //    stored after the flag, so that their stores do not hold back the downstream bricks
+    //    (worth it with few panels in flight; with many, the second staging costs more).
+    //    parts: 1 = face cells, 2 = the others, 3 = both
+    auto put = [&](const int parts) __attribute__((always_inline)) {
         const int wv = tid / 64, ln = tid & 63;
         double* stage = &xbuf[0][0] + wv * 16 * kPanel;  // this wave's 16 cells (2 KiB)
         const int col = ln & (kPanel - 1);
@@ -327,17 +334,19 @@ __global__ __launch_bounds__(kPT, 4) void k_panel(PanelArgs a) {
                     double* xp = a.x + (size_t)r * mm + p * kPanel + col;
                     const bool face =
                         ((w & 31) == 0 && I > 0) || (((w >> 5) & 31) == 0 && J > 0) || (((w >> 10) & 31) == 0 && K > 0);
-                    if (face) publish(xp, v);
-                    else *xp = v;
+                    if (face && (parts & 1)) publish(xp, v);
+                    else if (!face && (parts & 2)) *xp = v;
                 }
             }
             __builtin_amdgcn_s_waitcnt(0xC07F);  // the reads are done before the next slot's writes
             __builtin_amdgcn_wave_barrier();
         });
-    }
+    };
+    put(SPLIT ? 1 : 3);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) __hip_atomic_store(&a.flag[(size_t)lin * a.npanel + p], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (SPLIT) put(2);
 }
 
 // Records of a grid factor: per row its diagonal and the entries at +1, +nx, +nx*ny (0 where
@@ -460,12 +469,23 @@ int panel_solve(sptrsv_plan_s* P, int precision, const double* d_b, double* d_x,
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
     const unsigned g = (unsigned)items;
+    const bool split = npanel <= kSplitPanels;
     if (b888) {
-        if (vec) k_panel<8, 8, 8, true><<<g, kPT, 0, P->stream>>>(a);
-        else k_panel<8, 8, 8, false><<<g, kPT, 0, P->stream>>>(a);
+        if (split) {
+            if (vec) k_panel<8, 8, 8, true, true><<<g, kPT, 0, P->stream>>>(a);
+            else k_panel<8, 8, 8, false, true><<<g, kPT, 0, P->stream>>>(a);
+        } else {
+            if (vec) k_panel<8, 8, 8, true, false><<<g, kPT, 0, P->stream>>>(a);
+            else k_panel<8, 8, 8, false, false><<<g, kPT, 0, P->stream>>>(a);
+        }
     } else {
-        if (vec) k_panel<16, 16, 1, true><<<g, kPT, 0, P->stream>>>(a);
-        else k_panel<16, 16, 1, false><<<g, kPT, 0, P->stream>>>(a);
+        if (split) {
+            if (vec) k_panel<16, 16, 1, true, true><<<g, kPT, 0, P->stream>>>(a);
+            else k_panel<16, 16, 1, false, true><<<g, kPT, 0, P->stream>>>(a);
+        } else {
+            if (vec) k_panel<16, 16, 1, true, false><<<g, kPT, 0, P->stream>>>(a);
+            else k_panel<16, 16, 1, false, false><<<g, kPT, 0, P->stream>>>(a);
+        }
     }
     end_timed(P, stop);
     SPT_HIP(hipGetLastError());

@@ -69,6 +69,7 @@ int main(int argc, char** argv) {
     const int64_t items = (int64_t)L.blist.size() * npanel;
     std::vector<double> x((size_t)n * m, NAN), truth((size_t)n * m, NAN);
     std::vector<int64_t> done_item((size_t)L.blist.size() * npanel, -1);  // ticket that finished (brick, panel)
+    std::vector<unsigned char> face_done((size_t)n * npanel, 0);  // (row, panel) stored before its flag
     std::vector<int> brick_pos((size_t)L.blist.size());
     for (size_t q = 0; q < L.blist.size(); ++q) brick_pos[(size_t)L.blist[q]] = (int)q;
     std::vector<double> xbuf(2 * (kPT / 4) * kPanel), halo((size_t)kPanelMaxHalo * kPanel);
@@ -103,6 +104,8 @@ int main(int argc, char** argv) {
                 if (!ok) continue;
                 const int64_t row = gi + nx * (gj + ny * gk);
                 if (row < 0 || row >= n) return fail("halo row", row, h);
+                // the flag covers only the face stores: a halo cell must be one of them
+                if (!face_done[(size_t)row * npanel + p]) return fail("halo cell not stored before the flag", row, h);
                 halo_ok[h] = 1;
                 if (p * kPanel + col < m) halo[(size_t)h * kPanel + col] = x[(size_t)row * m + p * kPanel + col];
             }
@@ -209,6 +212,10 @@ int main(int argc, char** argv) {
                         const size_t at = (size_t)r * m + p * kPanel + col;
                         if (!std::isnan(x[at])) return fail("x written twice", r, col);
                         x[at] = stage[cl * kPanel + col];
+                        // the kernel's face test: local coordinate 0 towards a downstream brick
+                        const bool face = (panel_cell_i(c) == 0 && I > 0) || (panel_cell_j(c) == 0 && J > 0) ||
+                                          (panel_cell_k(c) == 0 && K > 0);
+                        if (face) face_done[(size_t)r * npanel + p] = 1;
                     }
             }
         done_item[(size_t)lin * npanel + p] = t;

@@ -46,7 +46,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("m", [1, 5, 16, 20, 48])
+@pytest.mark.parametrize("m", [1, 5, 16, 20, 48, 80])  # up to 4 panels: face stores flagged first
 @pytest.mark.parametrize("name", list(CASES))
 def test_panel_parity(name, m):
     U = CASES[name]()

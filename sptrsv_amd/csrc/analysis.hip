@@ -55,29 +55,79 @@ __global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict
     }
 }
 
-// Level of every row by a sync-free max-plus sweep: waves take 64-row tickets in descending row
-// order, loop after loop (persistent: a finished wave takes the next ticket instead of making way
-// for a new wave); a row's level is 1 + the deepest level among its off-diagonal columns (1 for
-// a diagonal-only row), exactly the reference's rule (src/core.cpp:244-270).  level 0 marks "not
-// yet known".  Each lane walks its row from the far end (the largest columns belong to rows
-// earlier in the sweep, which are known first), up to kLevelWin entries at a time: their columns
-// are read once per window and their levels are polled together, so a row waits one device round
-// trip after its last dependency is known, not one per entry.  A wave only ever waits on rows of
-// smaller tickets, held by waves that are running: the sweep cannot deadlock.
+// Tickets of the level sweep: 64 consecutive rows each, a wave's lane l on row hi - l.
+// Default order: descending rows (ticket t = rows n - 1 - 64 t down).  On a natural-order grid
+// factor, descending rows put about eight k-planes of 128^2 rows in flight, and each plane's
+// lines depend on one another in a chain; there the tickets go by line of the grid (segments
+// of 64 rows of one i-line, the upper segment first), lines by anti-diagonal distance
+// (ny - 1 - j) + (nz - 1 - k) from the last line, so that every line whose dependencies are
+// known can be in flight.  That order is used only after a check that every column of every
+// row lies in the row's own ticket or an earlier one (k_grid_order_check).
+struct LevelTickets {
+    int count;           // tickets
+    int nx, segs;        // grid order: row length of a line and 64-row segments per line
+    const int* line_of;  // grid order: line at each position (nullptr: descending rows)
+};
+
+__device__ __forceinline__ void ticket_rows(const LevelTickets& T, int n, int t, int* lo, int* hi) {
+    if (T.line_of == nullptr) {
+        *hi = n - 1 - 64 * t;
+        *lo = max(*hi - 63, 0);
+        return;
+    }
+    const int line = T.line_of[t / T.segs], seg = T.segs - 1 - t % T.segs;
+    *lo = line * T.nx + 64 * seg;
+    *hi = min(*lo + 63, line * T.nx + T.nx - 1);
+}
+
+// flag[0] = 1 if some column of some row lies in a later ticket of the grid order
+__global__ void k_grid_order_check(int n, const int* __restrict__ rowptr, const int* __restrict__ col, int nx,
+                                   int segs, const int* __restrict__ pos_of, int* flag) {
+    auto ticket = [&](int r) {
+        const int line = r / nx, i = r - line * nx;
+        return pos_of[line] * segs + (segs - 1 - i / 64);
+    };
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const int tr = ticket(r);
+        for (int p = rowptr[r] + 1; p < rowptr[r + 1]; ++p)
+            if (ticket(col[p]) > tr) {
+                atomicOr(flag, 1);
+                break;
+            }
+    }
+}
+
+// Level of every row by a sync-free max-plus sweep: waves take tickets (above) loop after loop
+// (persistent: a finished wave takes the next ticket instead of making way for a new wave); a
+// row's level is 1 + the deepest level among its off-diagonal columns (1 for a diagonal-only
+// row), exactly the reference's rule (src/core.cpp:244-270).  level 0 marks "not yet known".
+// Each lane walks its row from the far end, up to kLevelWin entries at a time.  Columns above
+// the ticket's rows ("external") lie in earlier tickets: their levels are polled from memory,
+// a window at a time.  Columns inside the ticket ("internal": consecutive rows of a grid line
+// form a chain) are read from the wave's level slots in LDS: between two device rounds the wave
+// runs LDS-only rounds until no lane progresses, so a chain inside a ticket costs LDS round
+// trips, not device ones.  A wave waits only on rows of earlier tickets, held by running waves,
+// and on its own lanes' rows (columns lie above their row): the sweep cannot deadlock.
 constexpr int kLevelWin = 8;
 
-__global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_levels(int n, LevelTickets T, const int* __restrict__ rowptr,
                                                 const int* __restrict__ col, int* levels,
                                                 int* ctrl, uint64_t limit) {
+    __shared__ int s_lv[256];
     const int lane = threadIdx.x & (kWave - 1);
+    volatile int* lv_own = s_lv + (threadIdx.x & ~(kWave - 1));  // this wave's 64 slots
     int mine = 0;
     Watchdog wd;
     wd.arm(&ctrl[1], limit);
     while (true) {
         const int t = wave_ticket(&ctrl[0]);
-        if (t * kWave >= n) break;
-        const int row = n - 1 - (t * kWave + lane);
-        bool done = row < 0;
+        if (t >= T.count) break;
+        int lo, hi;
+        ticket_rows(T, n, t, &lo, &hi);
+        const int row = hi - lane;
+        lv_own[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        bool done = row < lo;
         int first = 0, p = 0, deepest = 0;
         bool has = false;
         if (!done) {
@@ -85,40 +135,93 @@ __global__ __launch_bounds__(256) void k_levels(int n, const int* __restrict__ r
             p = rowptr[row + 1];
             has = p > first;
         }
+        // the current window: entries [p - cnt, p) with columns c[0] > c[1] > ... (largest
+        // first); c[w0 ..] not yet taken.  External columns form a prefix of the window.
         int c[kLevelWin];
-        int cnt = 0;  // entries in the current window [p - cnt, p); 0: columns not read yet
+        int cnt = 0, w0 = 0;  // cnt 0: columns not read yet
         int idle = 0;
+        auto load_window = [&]() {
+            cnt = min(kLevelWin, p - first);
+            w0 = 0;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) c[j] = col[max(p - cnt, p - 1 - j)];
+        };
+        auto ext_left = [&]() {
+            bool e = false;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) e = e || (j >= w0 && j < cnt && c[j] > hi);
+            return e;
+        };
+        auto consumed = [&](int mx) {
+            deepest = mx;
+            p -= cnt;
+            cnt = 0;
+            w0 = 0;
+        };
+        // device round: the window's external entries, polled from memory
+        auto take_ext = [&]() {
+            int lv[kLevelWin], ne = w0;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) {
+                const bool use = j >= w0 && j < cnt && c[j] > hi;
+                lv[j] = use ? __hip_atomic_load(&levels[c[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
+                ne = use ? j + 1 : ne;
+            }
+            bool all = true;
+            int mx = deepest;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) {
+                all = all && lv[j] != 0;
+                mx = lv[j] > mx ? lv[j] : mx;
+            }
+            if (all) {
+                if (ne >= cnt) consumed(mx);
+                else {
+                    deepest = mx;
+                    w0 = ne;
+                }
+            }
+            return all;
+        };
+        // LDS round: the rest of the window, all internal, from the wave's slots
+        auto take_int = [&]() {
+            bool all = true;
+            int mx = deepest;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j) {
+                const int v = (j >= w0 && j < cnt) ? lv_own[hi - c[j]] : 1;
+                all = all && v != 0;
+                mx = v > mx ? v : mx;
+            }
+            if (all) consumed(mx);
+            return all;
+        };
+        auto finish = [&]() {
+            const int lev = has ? 1 + deepest : 1;
+            lv_own[lane] = lev;
+            __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            mine = lev > mine ? lev : mine;
+            done = true;
+        };
         while (true) {
             bool progressed = false;
             if (!done && p > first) {
-                if (cnt == 0) {
-                    cnt = min(kLevelWin, p - first);
-#pragma unroll
-                    for (int j = 0; j < kLevelWin; ++j) c[j] = col[max(p - cnt, p - 1 - j)];
-                }
-                int lv[kLevelWin];
-#pragma unroll
-                for (int j = 0; j < kLevelWin; ++j)
-                    lv[j] = j < cnt ? __hip_atomic_load(&levels[c[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
-                bool all = true;
-                int mx = deepest;
-#pragma unroll
-                for (int j = 0; j < kLevelWin; ++j) {
-                    all = all && lv[j] != 0;
-                    mx = lv[j] > mx ? lv[j] : mx;
-                }
-                if (all) {
-                    deepest = mx;
-                    p -= cnt;
-                    cnt = 0;
-                    progressed = true;
-                }
+                if (cnt == 0) load_window();
+                if (ext_left()) progressed = take_ext();
             }
-            if (!done && p <= first) {
-                const int lev = has ? 1 + deepest : 1;
-                __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                mine = lev > mine ? lev : mine;
-                done = true;
+            // LDS rounds until no lane progresses
+            while (true) {
+                bool lp = false;
+                if (!done && p > first) {
+                    if (cnt == 0) load_window();
+                    if (!ext_left()) lp = take_int();
+                }
+                if (!done && p <= first) {
+                    finish();
+                    lp = true;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (!__any(lp)) break;
                 progressed = true;
             }
             if (__all(done)) break;
@@ -251,12 +354,37 @@ int analyse(sptrsv_plan_s* p) {
     if (p->depth_lb >= 2048 && (int64_t)n <= 256 * (int64_t)p->depth_lb) {
         SPT_CHECK(chain_levels(p, p->ticket));
     } else {
-        const int waves = (n + kWave - 1) / kWave;
+        LevelTickets T{(n + kWave - 1) / kWave, 0, 0, nullptr};
+        if (p->grid && p->grid_dims[0] * p->grid_dims[1] * p->grid_dims[2] == n) {
+            // lines of the grid by anti-diagonal, and each line's position, in the work buffer
+            // (free between the transpose and the level sort)
+            const int nx = (int)p->grid_dims[0], ny = (int)p->grid_dims[1], nz = (int)p->grid_dims[2];
+            const int lines = ny * nz, segs = (nx + 63) / 64;
+            std::vector<int> tab(2 * (size_t)lines);
+            int q = 0;
+            for (int d = 0; d <= ny - 1 + nz - 1; ++d)
+                for (int k = nz - 1; k >= 0; --k) {
+                    const int j = ny - 1 - (d - (nz - 1 - k));
+                    if (j < 0 || j >= ny) continue;
+                    tab[(size_t)q] = j + ny * k;
+                    tab[(size_t)lines + (size_t)(j + ny * k)] = q;
+                    ++q;
+                }
+            int* d_tab = static_cast<int*>(work.p);
+            SPT_HIP(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+            k_grid_order_check<<<grid_for(n), 256, 0, s>>>(n, p->rowptr, p->col, nx, segs, d_tab + lines,
+                                                           p->ticket + 3);
+            int bad = 0;
+            SPT_HIP(hipMemcpyAsync(&bad, p->ticket + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+            SPT_HIP(hipStreamSynchronize(s));
+            k_fill_int<<<1, 64, 0, s>>>(p->ticket + 3, 0, 1);
+            if (!bad) T = LevelTickets{lines * segs, nx, segs, d_tab};
+        }
         const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
         // persistent waves, 8 per CU (32 were slower: the polls of rows in flight contend), or
         // fewer for small factors
-        const int blocks = std::min((waves + 3) / 4, device_compute_units() * 2);
-        k_levels<<<blocks, 256, 0, s>>>(n, p->rowptr, p->col, p->levels, p->ticket, limit);
+        const int blocks = std::min((T.count + 3) / 4, device_compute_units() * 2);
+        k_levels<<<blocks, 256, 0, s>>>(n, T, p->rowptr, p->col, p->levels, p->ticket, limit);
     }
     int ctrl[3] = {0, 0, 0};
     SPT_HIP(hipMemcpyAsync(ctrl, p->ticket, sizeof(ctrl), hipMemcpyDeviceToHost, s));

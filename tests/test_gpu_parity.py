@@ -47,7 +47,30 @@ def small_matrices():
     vals = np.concatenate([np.full(n, 2.0), np.full(n - 1, -0.9)])
     o = np.lexsort((cols, rows))
     mats["chain_4000"] = gen._from_rows(n, rows[o], cols[o], vals[o], "chain")
+    # grid factors: lines of 130 rows (three 64-row segments of the level sweep, the last one
+    # short) with a tenth of the entries missing; and offsets {1, 8, 64} with every entry present,
+    # wrap-arounds included, where the sweep's line order is not topological (row order instead)
+    mats["holes_130x5x4"] = _offsets(130 * 5 * 4, (1, 130, 650), 7, nx=130, ny=5, drop=0.1)
+    mats["wrap_8x8x8"] = _offsets(512, (1, 8, 64), 8)
     return mats
+
+
+def _offsets(n, offsets, seed, nx=0, ny=0, drop=0.0):
+    """entries (r, r + o) for o in offsets; with nx, ny none crossing a line or plane edge and
+    a fraction `drop` removed; diagonal in [2, 3], off-diagonals in [-0.3, 0.3]"""
+    rng = np.random.default_rng(seed)
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for o in offsets:
+        r = np.arange(n - o)
+        if nx:
+            i, j = r % nx, (r // nx) % ny
+            r = r[~(((o == 1) & (i == nx - 1)) | ((o == nx) & (j == ny - 1))) & (rng.random(len(r)) >= drop)]
+        rows.append(r)
+        cols.append(r + o)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    vals = np.where(rows == cols, rng.uniform(2, 3, len(rows)), rng.uniform(-0.3, 0.3, len(rows)))
+    o = np.lexsort((cols, rows))
+    return gen._from_rows(n, rows[o], cols[o], vals[o], "offsets")
 
 
 MATS = small_matrices()

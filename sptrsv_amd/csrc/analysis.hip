@@ -58,24 +58,25 @@ __global__ void k_gather_csc(const int* __restrict__ perm, const int* __restrict
 // Tickets of the level sweep: 64 consecutive rows each, a wave's lane l on row hi - l.
 // Default order: descending rows (ticket t = rows n - 1 - 64 t down).  On a natural-order grid
 // factor, descending rows put about eight k-planes of 128^2 rows in flight, and each plane's
-// lines depend on one another in a chain; there the tickets go by line of the grid (segments
-// of 64 rows of one i-line, the upper segment first), lines by anti-diagonal distance
-// (ny - 1 - j) + (nz - 1 - k) from the last line, so that every line whose dependencies are
-// known can be in flight.  That order is used only after a check that every column of every
-// row lies in the row's own ticket or an earlier one (k_grid_order_check).
+// lines depend on one another in a chain; there a ticket is a segment of 64 rows of one i-line
+// (segment g = i / 64), and tickets go by their anti-diagonal distance
+// (segs - 1 - g) + (ny - 1 - j) + (nz - 1 - k) from the last segment, so that every segment
+// whose dependencies are known can be in flight and each distance costs one device round trip.
+// That order is used only after a check that every column of every row lies in the row's own
+// ticket or an earlier one (k_grid_order_check).
 struct LevelTickets {
-    int count;           // tickets
-    int nx, segs;        // grid order: row length of a line and 64-row segments per line
-    const int* line_of;  // grid order: line at each position (nullptr: descending rows)
+    int count;         // tickets
+    int nx, segs;      // grid order: row length of a line and 64-row segments per line
+    const int* item;   // grid order: line * segs + segment at each position (nullptr: descending rows)
 };
 
 __device__ __forceinline__ void ticket_rows(const LevelTickets& T, int n, int t, int* lo, int* hi) {
-    if (T.line_of == nullptr) {
+    if (T.item == nullptr) {
         *hi = n - 1 - 64 * t;
         *lo = max(*hi - 63, 0);
         return;
     }
-    const int line = T.line_of[t / T.segs], seg = T.segs - 1 - t % T.segs;
+    const int e = T.item[t], line = e / T.segs, seg = e - line * T.segs;
     *lo = line * T.nx + 64 * seg;
     *hi = min(*lo + 63, line * T.nx + T.nx - 1);
 }
@@ -85,7 +86,7 @@ __global__ void k_grid_order_check(int n, const int* __restrict__ rowptr, const 
                                    int segs, const int* __restrict__ pos_of, int* flag) {
     auto ticket = [&](int r) {
         const int line = r / nx, i = r - line * nx;
-        return pos_of[line] * segs + (segs - 1 - i / 64);
+        return pos_of[line * segs + i / 64];
     };
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         const int tr = ticket(r);
@@ -196,6 +197,31 @@ __global__ __launch_bounds__(256) void k_levels(int n, LevelTickets T, const int
             if (all) consumed(mx);
             return all;
         };
+        // "simple" rows (the whole row in one window, no internal column but row + 1): their
+        // level is max(E, level(row + 1) + 1) with E = 1 + the deepest external level, so along a
+        // run of them level - lane is a running maximum of E - lane, which a segmented prefix
+        // scan over the wave resolves at once (a grid line's chain in six shuffle steps)
+        bool simple = false, link = false;
+        if (!done) {
+            const bool whole = p - first <= kLevelWin;
+            if (has) load_window();
+            int nint = 0;
+            bool next_only = true;
+#pragma unroll
+            for (int j = 0; j < kLevelWin; ++j)
+                if (j < cnt && c[j] <= hi) {
+                    ++nint;
+                    next_only = next_only && c[j] == row + 1;
+                }
+            simple = whole && (nint == 0 || (nint == 1 && next_only));
+            link = simple && nint == 1;
+        }
+        auto finish_at = [&](int lev) {
+            lv_own[lane] = lev;
+            __hip_atomic_store(&levels[row], lev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            mine = lev > mine ? lev : mine;
+            done = true;
+        };
         auto finish = [&]() {
             const int lev = has ? 1 + deepest : 1;
             lv_own[lane] = lev;
@@ -208,6 +234,27 @@ __global__ __launch_bounds__(256) void k_levels(int n, LevelTickets T, const int
             if (!done && p > first) {
                 if (cnt == 0) load_window();
                 if (ext_left()) progressed = take_ext();
+            }
+            // scan round: key = level - lane (known) or E - lane (simple, externals taken);
+            // a lane starts a segment unless it links to lane - 1; lanes of unknown level poison
+            // their segment
+            if (__any(simple && !done)) {
+                const bool ready = simple && !done && (p <= first || !ext_left());
+                int key = done ? lv_own[lane] - lane : (has ? 1 + deepest : 1) - lane;
+                int fl = (done || !ready || !link ? 1 : 0) | (!done && !ready ? 2 : 0);  // 1 start, 2 poison
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const int k2 = __shfl_up(key, d, kWave), f2 = __shfl_up(fl, d, kWave);
+                    if (lane >= d && !(fl & 1)) {
+                        key = key > k2 ? key : k2;
+                        fl |= f2;
+                    }
+                }
+                if (ready && !(fl & 2)) {
+                    finish_at(key + lane);
+                    progressed = true;
+                }
+                __builtin_amdgcn_wave_barrier();
             }
             // LDS rounds until no lane progresses
             while (true) {
@@ -337,7 +384,8 @@ int analyse(sptrsv_plan_s* p) {
     const size_t zn = align_up((size_t)std::max(n, 1) * sizeof(int));
     const size_t level_bytes = zn + sort_scratch_bytes(n) + scan_scratch_bytes((int64_t)n + 1);
     DevBuf work;
-    SPT_HIP(hipMalloc(&work.p, std::max(transpose_scratch_bytes(nnz), level_bytes)));
+    const size_t work_bytes = std::max(transpose_scratch_bytes(nnz), level_bytes);
+    SPT_HIP(hipMalloc(&work.p, work_bytes));
     SPT_CHECK(device_transpose(n, nnz, p->rowptr, p->col, p->val, p->colptr, p->rowidx, p->cval, s, work.p));
 
     // ---- in-degree: entries per row ------------------------------------------------------
@@ -355,30 +403,36 @@ int analyse(sptrsv_plan_s* p) {
         SPT_CHECK(chain_levels(p, p->ticket));
     } else {
         LevelTickets T{(n + kWave - 1) / kWave, 0, 0, nullptr};
-        if (p->grid && p->grid_dims[0] * p->grid_dims[1] * p->grid_dims[2] == n) {
-            // lines of the grid by anti-diagonal, and each line's position, in the work buffer
-            // (free between the transpose and the level sort)
+        if (p->grid && p->grid_dims[0] * p->grid_dims[1] * p->grid_dims[2] == n &&
+            2 * p->grid_dims[1] * p->grid_dims[2] * ((p->grid_dims[0] + 63) / 64) * (int64_t)sizeof(int) <=
+                (int64_t)work_bytes) {
+            // line segments by anti-diagonal distance (items line * segs + segment), and each
+            // item's position, in the work buffer (free between the transpose and the level sort)
             const int nx = (int)p->grid_dims[0], ny = (int)p->grid_dims[1], nz = (int)p->grid_dims[2];
-            const int lines = ny * nz, segs = (nx + 63) / 64;
-            std::vector<int> tab(2 * (size_t)lines);
-            int q = 0;
-            for (int d = 0; d <= ny - 1 + nz - 1; ++d)
-                for (int k = nz - 1; k >= 0; --k) {
-                    const int j = ny - 1 - (d - (nz - 1 - k));
-                    if (j < 0 || j >= ny) continue;
-                    tab[(size_t)q] = j + ny * k;
-                    tab[(size_t)lines + (size_t)(j + ny * k)] = q;
-                    ++q;
-                }
+            const int segs = (nx + 63) / 64, items = ny * nz * segs;
+            std::vector<int> tab(2 * (size_t)items);
+            std::vector<int> start((size_t)(segs + ny + nz), 0);  // counting sort by distance
+            auto dist = [&](int g, int j, int k) { return (segs - 1 - g) + (ny - 1 - j) + (nz - 1 - k); };
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int g = 0; g < segs; ++g) ++start[(size_t)dist(g, j, k) + 1];
+            for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+            for (int k = nz - 1; k >= 0; --k)
+                for (int j = ny - 1; j >= 0; --j)
+                    for (int g = segs - 1; g >= 0; --g) {
+                        const int e = (j + ny * k) * segs + g, q = start[(size_t)dist(g, j, k)]++;
+                        tab[(size_t)q] = e;
+                        tab[(size_t)items + (size_t)e] = q;
+                    }
             int* d_tab = static_cast<int*>(work.p);
             SPT_HIP(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
-            k_grid_order_check<<<grid_for(n), 256, 0, s>>>(n, p->rowptr, p->col, nx, segs, d_tab + lines,
+            k_grid_order_check<<<grid_for(n), 256, 0, s>>>(n, p->rowptr, p->col, nx, segs, d_tab + items,
                                                            p->ticket + 3);
             int bad = 0;
             SPT_HIP(hipMemcpyAsync(&bad, p->ticket + 3, sizeof(int), hipMemcpyDeviceToHost, s));
             SPT_HIP(hipStreamSynchronize(s));
             k_fill_int<<<1, 64, 0, s>>>(p->ticket + 3, 0, 1);
-            if (!bad) T = LevelTickets{lines * segs, nx, segs, d_tab};
+            if (!bad) T = LevelTickets{items, nx, segs, d_tab};
         }
         const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
         // persistent waves, 8 per CU (32 were slower: the polls of rows in flight contend), or

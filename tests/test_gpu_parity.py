@@ -613,6 +613,31 @@ def test_supernodal_given_partition(m):
         assert O.rel_error(xs if m == 1 else xs.T, x) <= FP64_TOL
 
 
+def test_supernodal_one_column_kernels():
+    """One right-hand side on supernodes of banded factors (the one-column kernel): a single
+    band (its supernodes form one chain) and six independent bands side by side (a supernode
+    DAG six wide), both against the oracle, and the same x on a second solve."""
+    one = gen.laplace2d_cholesky(30)
+    blocks = [gen.laplace2d_cholesky(12) for _ in range(6)]
+    rows, cols, vals, off = [], [], [], 0
+    for B in blocks:
+        r = np.repeat(np.arange(B.n), np.diff(B.rowptr))
+        rows.append(r + off)
+        cols.append(B.col.astype(np.int64) + off)
+        vals.append(B.val)
+        off += B.n
+    six = gen._from_rows(off, np.concatenate(rows), np.concatenate(cols).astype(np.int32), np.concatenate(vals), "six")
+    for name, U in (("band", one), ("six bands", six)):
+        with capi.Plan.from_upper(U) as P:
+            sn = P.build_supernodes()
+            assert len(sn) > 2, name
+            xt, b = gen.rhs_for(U, 1, seed=12)
+            x = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+            assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, name
+            x2 = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
+            assert np.array_equal(x, x2), name  # fixed summation order
+
+
 def test_supernodal_detected_and_split():
     U = gen.random_supernodal(16, seed=8, wmin=100, wmax=300)  # wider than 128 -> split
     with capi.Plan.from_upper(U) as P:

@@ -355,51 +355,40 @@ __global__ void k_reset_ticket_sn(int* ctrl) {
     if (threadIdx.x == 0) ctrl[0] = 0;
 }
 
-// Wave 0 of a workgroup waits (relaxed polls, watchdog) until the flags of the supernodes owning
-// the rectangular columns of supernode k, column panel `panel`, hold this solve's epoch.  The
-// caller follows with a workgroup barrier and an acquire fence.
-__device__ __forceinline__ void sn_wait_deps(const SnArgs& a, int k, int panel, int lane) {
-    Watchdog wd;
-    wd.arm(&a.ctrl[1], a.limit);
-    const int d1 = a.dep_ptr[k + 1];
-    int idle = 0;
-    bool give_up = false;
-    for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
-        const int qd = base + lane;
-        bool ready = qd >= d1;
-        const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[qd] * a.npanel + panel];
-        while (!__all(ready)) {
-            if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
-            if (__all(ready)) break;
-            __builtin_amdgcn_s_sleep(1);
-            if ((++idle & 63) == 0 && __any(wd.expired())) {
-                give_up = true;
-                break;
-            }
-        }
-    }
-}
-
 // One right-hand side.  The MFMA kernel's 16-column tiles would leave 15 of 16 columns idle, and
 // it loads R and Tinv after its wait, one memory latency per chunk; on a chain of supernodes
 // (a banded factor in blocks of 128 rows) those latencies add up along the whole chain.  Here a
 // workgroup of 512 threads per supernode gives each row four lanes (lane q of row i = t / 4
 // takes columns q, q + 4, q + 8, ...) and loads everything that does not depend on X before it
-// waits: b_s, the 4 * kSgPre or fewer columns of R_s with their column indices, and row i of Tinv_s
-// (columns j >= i).  After the wait it gathers X at those columns, forms
-//     W_i   = b_i - R_s(i, :) X(rect)      four partial sums, reduced across the quad,
-//     X_s,i = Tinv_s(i, :) W               W passed through LDS, reduced the same way,
-// and publishes X_s with the flag protocol of k_sn_solve (panel 0).  Every rectangular column
-// is loaded before the wait, so the kernel needs r <= 4 * kSgPre for every supernode (a band of
-// width <= 128 in blocks of 128 rows); the solve takes the MFMA kernel otherwise.  The sums run
-// in a fixed order: the same inputs give the same X on every run.
+// waits: b_s, the 4 * kSgPre or fewer columns of R_s with their column indices, and row i of
+// Tinv_s (columns j >= i).  x is self-publishing (filled with the sentinel before the launch,
+// each value stored once with a relaxed device-scope store), so there is no flag: each lane
+// polls x at its columns and, as each value arrives, turns its R entry into the product; once
+// all have arrived
+//     W_i   = b_i - sum of the products      in a fixed order, four partial sums per row
+//                                            reduced across the quad,
+//     X_s,i = Tinv_s(i, :) W                 W passed through LDS, reduced the same way,
+// and X_s is published.  Every rectangular column is loaded before the wait, so the kernel
+// needs r <= 4 * kSgPre for every supernode (a band of width <= 128 in blocks of 128 rows);
+// the solve takes the MFMA kernel otherwise, and when x overlaps b (the fill would destroy b).
+// The sums run in a fixed order: the same inputs give the same X on every run.
 constexpr int kSgPre = 32;  // R columns per lane loaded before the wait
 constexpr int kSgT = kMaxW / 4;  // Tinv columns per lane
 
+// x := sentinel, and the ticket counter reset
+__global__ void k_fill_sentinel_sn(double* x, int64_t n, int* ctrl) {
+    const uint64_t s = Bits<double>::sentinel;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[0] = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        reinterpret_cast<uint64_t*>(x)[i] = s;
+}
+
 __global__ __launch_bounds__(512, 1) void k_sn_gemv(SnArgs a) {
+    using U = uint64_t;
+    constexpr U S = Bits<double>::sentinel;
     __shared__ double sW[kMaxW];
     __shared__ int s_ticket;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x;
     if (tid == 0) s_ticket = atomicAdd(&a.ctrl[0], 1);
     __syncthreads();
     const int k = a.sn - 1 - s_ticket;
@@ -411,7 +400,7 @@ __global__ __launch_bounds__(512, 1) void k_sn_gemv(SnArgs a) {
     const int* C = a.cols + a.offc[k];
     const double* Ti = a.tinv + a.offt[k] + (size_t)(row ? i : 0) * w;
 
-    // ---- independent of X: b, R (first columns), column indices, Tinv row ---------------
+    // ---- independent of X: b, R, column indices, Tinv row ---------------------------------
     const double bi = (row && q == 0) ? a.b[s0 + i] : 0.0;
     double rv[kSgPre], tv[kSgT];
     int cv[kSgPre];
@@ -428,19 +417,37 @@ __global__ __launch_bounds__(512, 1) void k_sn_gemv(SnArgs a) {
         tv[u] = (row && j >= i && j < w) ? Ti[j] : 0.0;
     }
 
-    // ---- wait for the supernodes owning the rectangular columns --------------------------
-    if (wv == 0) sn_wait_deps(a, k, 0, lane);
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // ---- x at the rectangular columns, as it is published: rv[u] becomes the product -------
+    uint32_t left = 0;  // bit u: column u still to arrive
+#pragma unroll
+    for (int u = 0; u < kSgPre; ++u) left |= cv[u] >= 0 ? 1u << u : 0u;
+    {
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        int idle = 0;
+        while (__any(left != 0)) {
+#pragma unroll
+            for (int u = 0; u < kSgPre; ++u) {
+                if (left & (1u << u)) {
+                    const U v = poll_bits(a.x + cv[u]);
+                    if (v != S) {
+                        rv[u] *= from_bits<double>(v);
+                        left &= ~(1u << u);
+                    }
+                }
+            }
+            if (!__any(left != 0)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if ((++idle & 63) == 0 && __any(left != 0 && wd.expired())) break;
+        }
+    }
 
     // ---- W = b - R X(rect) -----------------------------------------------------------------
     double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
     for (int u = 0; u < kSgPre; u += 2) {
-        const double x0 = cv[u] >= 0 ? a.x[cv[u]] : 0.0;
-        const double x1 = cv[u + 1] >= 0 ? a.x[cv[u + 1]] : 0.0;
-        acc0 = fma(rv[u], x0, acc0);
-        acc1 = fma(rv[u + 1], x1, acc1);
+        acc0 += rv[u];
+        acc1 += rv[u + 1];
     }
     double s = acc0 + acc1;
     s += __shfl_xor(s, 1, 4);
@@ -458,10 +465,7 @@ __global__ __launch_bounds__(512, 1) void k_sn_gemv(SnArgs a) {
     double xs = t0 + t1;
     xs += __shfl_xor(xs, 1, 4);
     xs += __shfl_xor(xs, 2, 4);
-    if (row && q == 0) a.x[s0 + i] = xs;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&a.flag[k], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (row && q == 0) publish(a.x + s0 + i, xs);
 }
 
 template <typename X>
@@ -797,7 +801,8 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     // average (chain-like: the one-workgroup-per-CU residency of k_sn_gemv costs nothing there)
     // and whose rectangular blocks fit its prefetch: k_sn_gemv.  Else k_sn_solve, NW waves per
     // panel of NW 16-column blocks; its one-wave form keeps four supernodes per CU resident.
-    const bool gemv = m == 1 && P->sn_rmax <= 4 * kSgPre && P->sn <= 256 * (int64_t)P->sn_levels;
+    const bool apart = d_x + (size_t)P->n * m <= d_b || d_b + (size_t)P->n * m <= d_x;
+    const bool gemv = m == 1 && apart && P->sn_rmax <= 4 * kSgPre && P->sn <= 256 * (int64_t)P->sn_levels;
     const int nw = sn_waves(nblk);
     const int npanel = gemv ? 1 : (nblk + nw - 1) / nw;
     SPT_CHECK(ensure_sn_flags(P, npanel));
@@ -805,7 +810,9 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     SnArgs a{P->sn, m, npanel, P->sn_start, P->sn_r, P->sn_offt, P->sn_offr, P->sn_offc, P->sn_cols, P->sn_tinv,
              P->sn_rect, P->sn_tri, P->sn_dep_ptr, P->sn_dep, P->sn_flag, P->sn_epoch, d_b, d_x, nullptr,
              P->ticket, limit};
-    k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
+    if (gemv) k_fill_sentinel_sn<<<(unsigned)std::min<int64_t>(((int64_t)P->n + 255) / 256, 65536), 256, 0, P->stream>>>(
+        d_x, P->n, P->ticket);
+    else k_reset_ticket_sn<<<1, 64, 0, P->stream>>>(P->ticket);
     hipEvent_t stop;
     SPT_CHECK(begin_timed(P, &stop));
     const unsigned grid = (unsigned)((int64_t)P->sn * npanel);

@@ -801,7 +801,8 @@ int supernodal_solve(sptrsv_plan_s* P, int precision, const double* d_b, double*
     // average (chain-like: the one-workgroup-per-CU residency of k_sn_gemv costs nothing there)
     // and whose rectangular blocks fit its prefetch: k_sn_gemv.  Else k_sn_solve, NW waves per
     // panel of NW 16-column blocks; its one-wave form keeps four supernodes per CU resident.
-    const bool apart = d_x + (size_t)P->n * m <= d_b || d_b + (size_t)P->n * m <= d_x;
+    const uintptr_t xb = (uintptr_t)d_x, bb = (uintptr_t)d_b, len = (uintptr_t)P->n * m * sizeof(double);
+    const bool apart = xb + len <= bb || bb + len <= xb;  // the sentinel fill of x must not touch b
     const bool gemv = m == 1 && apart && P->sn_rmax <= 4 * kSgPre && P->sn <= 256 * (int64_t)P->sn_levels;
     const int nw = sn_waves(nblk);
     const int npanel = gemv ? 1 : (nblk + nw - 1) / nw;

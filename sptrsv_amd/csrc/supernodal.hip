@@ -122,6 +122,43 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : NW == 8 ? 1 : 8 / NW) void k
     const int panel = t % a.npanel;
     if (k < 0) return;
     const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
+    const int c16 = lane & 15, quad = lane >> 4;
+    const size_t m = (size_t)a.m;
+    const int jj = 16 * (panel * NW + wv) + c16;
+    const bool jv = jj < a.m;
+    const int nI = (w + 15) >> 4;
+    v4d acc[8];
+#pragma unroll
+    for (int I = 0; I < 8; ++I) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 16 * I + quad + 4 * v;
+            acc[I][v] = (I < nI && i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
+        }
+    }
+
+    // ---- W = B_s - A X(cols) over K segments (solve: R_s; residual: T_s, then R_s) --------
+    // One wave (NW == 1) has registers to spare at its occupancy: it keeps TWO chunks of R and
+    // X in flight (sets A and B) and waits for a chunk one iteration after issuing it.
+    constexpr bool kDeep = NW == 1;
+    double rreg[kPerR], rreg2[kDeep ? kPerR : 1];
+    constexpr int KS = kSnKC / 4;  // MFMA K-steps per chunk
+    double bv[KS], bn[KS];
+    // The solve's first chunks of R (one, or two with one wave) do not depend on X: they load
+    // while wave 0 waits for the flags.
+    bool preloaded = !RESID;
+    if (!RESID) {
+        const double* Rk = a.rect + a.offr[k];
+        auto pre_r = [&](double* rr, int kg) {
+#pragma unroll
+            for (int q = 0; q < kPerR; ++q) {
+                const int e = tid + q * NT, row = e / kSnKC, kc = kg + e % kSnKC;
+                rr[q] = (row < w && kc < r) ? Rk[(size_t)row * r + kc] : 0.0;
+            }
+        };
+        pre_r(rreg, 0);
+        if (kDeep && min(kSnCChunk, r) > kSnKC) pre_r(rreg2, kSnKC);
+    }
     if (!RESID && wv == 0) {
         Watchdog wd;
         wd.arm(&a.ctrl[1], a.limit);
@@ -146,28 +183,6 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : NW == 8 ? 1 : 8 / NW) void k
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    const int c16 = lane & 15, quad = lane >> 4;
-    const size_t m = (size_t)a.m;
-    const int jj = 16 * (panel * NW + wv) + c16;
-    const bool jv = jj < a.m;
-    const int nI = (w + 15) >> 4;
-    v4d acc[8];
-#pragma unroll
-    for (int I = 0; I < 8; ++I) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = 16 * I + quad + 4 * v;
-            acc[I][v] = (I < nI && i < w && jv) ? a.b[(size_t)(s0 + i) * m + jj] : 0.0;
-        }
-    }
-
-    // ---- W = B_s - A X(cols) over K segments (solve: R_s; residual: T_s, then R_s) --------
-    // One wave (NW == 1) has registers to spare at its occupancy: it keeps TWO chunks of R and
-    // X in flight (sets A and B) and waits for a chunk one iteration after issuing it.
-    constexpr bool kDeep = NW == 1;
-    double rreg[kPerR], rreg2[kDeep ? kPerR : 1];
-    constexpr int KS = kSnKC / 4;  // MFMA K-steps per chunk
-    double bv[KS], bn[KS];
     // one K segment: A is w x K row-major (leading dimension lda); column kk of the segment is
     // X row sC[kk] (cols != nullptr, staged in LDS by blocks) or row s0 + kk (cols == nullptr)
     auto segment = [&](const double* A, int lda, int K, const int* cols) {
@@ -210,11 +225,12 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : NW == 8 ? 1 : 8 / NW) void k
                 // in flight in rb / xb; the iteration multiplies c, stores c + 1 to LDS, and
                 // issues c + 2 into the set c used.
                 double xa[KS], xb[KS];
+                const bool have = preloaded && cb == 0;  // chunks 0 and 1 loaded before the wait
                 load_x(xa, 0);
-                load_r_to(rreg, cb);
+                if (!have) load_r_to(rreg, cb);
                 if (nch > 1) {
                     load_x(xb, kSnKC);
-                    load_r_to(rreg2, cb + kSnKC);
+                    if (!have) load_r_to(rreg2, cb + kSnKC);
                 }
                 store_r_from(rreg, sbuf);
                 lds_barrier();
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : NW == 8 ? 1 : 8 / NW) void k
                 continue;
             }
             load_x(bn, 0);
-            load_r(cb);
+            if (!(preloaded && cb == 0)) load_r(cb);  // chunk 0 loaded before the wait
             store_r(sbuf);
             __syncthreads();
             for (int c = 0; c < nch; ++c) {

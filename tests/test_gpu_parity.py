@@ -51,6 +51,7 @@ def small_matrices():
     # short) with a tenth of the entries missing; and offsets {1, 8, 64} with every entry present,
     # wrap-arounds included, where the sweep's line order is not topological (row order instead)
     mats["holes_130x5x4"] = _offsets(130 * 5 * 4, (1, 130, 650), 7, nx=130, ny=5, drop=0.1)
+    mats["holes_100x30"] = _offsets(100 * 30, (1, 100), 9, nx=100, ny=30, drop=0.1)  # a 2D grid
     mats["wrap_8x8x8"] = _offsets(512, (1, 8, 64), 8)
     return mats
 

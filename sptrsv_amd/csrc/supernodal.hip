@@ -588,30 +588,6 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve_low(
     const int panel = t % a.npanel;
     if (k < 0) return;
     const int s0 = a.start[k], w = a.start[k + 1] - s0, r = a.rcount[k];
-    if (wv == 0) {
-        Watchdog wd;
-        wd.arm(&a.ctrl[1], a.limit);
-        const int d1 = a.dep_ptr[k + 1];
-        int idle = 0;
-        bool give_up = false;
-        for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
-            const int q = base + lane;
-            bool ready = q >= d1;
-            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.npanel + panel];
-            while (!__all(ready)) {
-                if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
-                if (__all(ready)) break;
-                __builtin_amdgcn_s_sleep(2);
-                if ((++idle & 63) == 0 && __any(wd.expired())) {
-                    give_up = true;
-                    break;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-
     const int c16 = lane & 15, quad = lane >> 4;
     const size_t m = (size_t)a.m;
     const int jj = 16 * (panel * NW + wv) + c16;
@@ -640,6 +616,34 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve_low(
             if (g / G < kMaxW) *reinterpret_cast<v4s*>(dst + (g / G) * LDR + 4 * (g % G)) = rreg[q];
         }
     };
+    // independent of X: the first chunk of R16 and the first block of column indices load
+    // while wave 0 waits for the flags
+    load_r(0);
+    for (int q = tid; q < min(kSnCChunk, r); q += NT) sC[q] = C[q];
+    if (wv == 0) {
+        Watchdog wd;
+        wd.arm(&a.ctrl[1], a.limit);
+        const int d1 = a.dep_ptr[k + 1];
+        int idle = 0;
+        bool give_up = false;
+        for (int base = a.dep_ptr[k]; base < d1 && !give_up; base += kWave) {
+            const int q = base + lane;
+            bool ready = q >= d1;
+            const int* f = ready ? nullptr : &a.flag[(size_t)a.dep[q] * a.npanel + panel];
+            while (!__all(ready)) {
+                if (!ready) ready = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+                if (__all(ready)) break;
+                __builtin_amdgcn_s_sleep(2);
+                if ((++idle & 63) == 0 && __any(wd.expired())) {
+                    give_up = true;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
     for (int cb = 0; cb < r; cb += kSnCChunk) {
         const int cn = min(kSnCChunk, r - cb);
         const int nch = (cn + KC - 1) / KC;
@@ -653,11 +657,13 @@ __global__ __launch_bounds__(64 * NW, NW == 1 ? 1 : 8 / NW) void k_sn_solve_low(
                 }
             }
         };
-        __syncthreads();
-        for (int q = tid; q < cn; q += NT) sC[q] = C[cb + q];
-        __syncthreads();
+        if (cb > 0) {  // the first block's indices and chunk were loaded before the wait
+            __syncthreads();
+            for (int q = tid; q < cn; q += NT) sC[q] = C[cb + q];
+            __syncthreads();
+        }
         load_x(0);
-        load_r(cb);
+        if (cb > 0) load_r(cb);
         store_r(sR);
         __syncthreads();
         for (int c = 0; c < nch; ++c) {

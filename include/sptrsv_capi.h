@@ -208,7 +208,8 @@ int sptrsv_selector_set(int in, int h1, int h2, int out, const float* params, co
                         const int* classes);
 int sptrsv_selector_clear(void);
 
-/* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64).
+/* Solve U X = B for m right-hand sides.  d_b, d_x: device, row-major n x m (FP64), not
+ * overlapping (most kernels mark x's unknown values in x itself before they solve).
  * Asynchronous on the plan's stream. */
 int sptrsv_solve(sptrsv_plan_t plan, const sptrsv_params_t* params, const double* d_b,
                  double* d_x, int m);

@@ -617,7 +617,7 @@ def test_supernodal_given_partition(m):
 def test_supernodal_one_column_kernels():
     """One right-hand side on supernodes of banded factors (the one-column kernel): a single
     band (its supernodes form one chain) and six independent bands side by side (a supernode
-    DAG six wide), both against the oracle, the same x on a second solve, and in place."""
+    DAG six wide), both against the oracle, the same x on a second solve, and with x over b."""
     one = gen.laplace2d_cholesky(30)
     blocks = [gen.laplace2d_cholesky(12) for _ in range(6)]
     rows, cols, vals, off = [], [], [], 0
@@ -637,7 +637,9 @@ def test_supernodal_one_column_kernels():
             assert O.rel_error(oracle_solve(U, b), x) <= FP64_TOL, name
             x2 = P.solve_host(b, capi.params(variant=capi.VAR_SUPERNODAL))
             assert np.array_equal(x, x2), name  # fixed summation order
-            d = capi.DeviceArray.from_host(b)  # in place (x over b): the MFMA kernel, no fill
+            # x over b (outside sptrsv_solve's contract): the one-column kernel, whose fill would
+            # destroy b, is not taken; the MFMA kernel solves in place
+            d = capi.DeviceArray.from_host(b)
             P.solve(d, d, 1, capi.params(variant=capi.VAR_SUPERNODAL))
             P.sync()
             assert O.rel_error(oracle_solve(U, b), d.download()) <= FP64_TOL, name

@@ -403,6 +403,7 @@ int analyse(sptrsv_plan_s* p) {
         SPT_CHECK(chain_levels(p, p->ticket));
     } else {
         LevelTickets T{(n + kWave - 1) / kWave, 0, 0, nullptr};
+        int widest = 0;  // grid order: most segments at one anti-diagonal distance
         if (p->grid && p->grid_dims[0] * p->grid_dims[1] * p->grid_dims[2] == n &&
             2 * p->grid_dims[1] * p->grid_dims[2] * ((p->grid_dims[0] + 63) / 64) * (int64_t)sizeof(int) <=
                 (int64_t)work_bytes) {
@@ -416,6 +417,7 @@ int analyse(sptrsv_plan_s* p) {
             for (int k = 0; k < nz; ++k)
                 for (int j = 0; j < ny; ++j)
                     for (int g = 0; g < segs; ++g) ++start[(size_t)dist(g, j, k) + 1];
+            for (size_t d = 1; d < start.size(); ++d) widest = std::max(widest, start[d]);
             for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
             for (int k = nz - 1; k >= 0; --k)
                 for (int j = ny - 1; j >= 0; --j)
@@ -435,10 +437,17 @@ int analyse(sptrsv_plan_s* p) {
             if (!bad) T = LevelTickets{items, nx, segs, d_tab};
         }
         const uint64_t limit = p->ticks_per_ms * 60000ull;  // a level sweep never waits a minute
-        // persistent waves, 8 per CU (32 were slower: the polls of rows in flight contend), or
-        // fewer for small factors
-        const int blocks = std::min((T.count + 3) / 4, device_compute_units() * 2);
-        k_levels<<<blocks, 256, 0, s>>>(n, T, p->rowptr, p->col, p->levels, p->ticket, limit);
+        // persistent waves.  Row order: 8 per CU (32 were slower: the polls of rows in flight
+        // contend), fewer for small factors.  Grid order: about half the widest anti-diagonal,
+        // one wave per workgroup -- on 128^3 (256 segments at the widest) 8, 4, 2, 1 and 0.5
+        // waves per CU took 1.81, 1.53, 1.41, 1.27 and 1.19 ms, 0.25 per CU 1.89 ms
+        if (T.item) {
+            const int waves = std::min(T.count, std::max(32, std::min(widest / 2, device_compute_units())));
+            k_levels<<<waves, 64, 0, s>>>(n, T, p->rowptr, p->col, p->levels, p->ticket, limit);
+        } else {
+            const int blocks = std::min((T.count + 3) / 4, device_compute_units() * 2);
+            k_levels<<<blocks, 256, 0, s>>>(n, T, p->rowptr, p->col, p->levels, p->ticket, limit);
+        }
     }
     int ctrl[3] = {0, 0, 0};
     SPT_HIP(hipMemcpyAsync(ctrl, p->ticket, sizeof(ctrl), hipMemcpyDeviceToHost, s));
